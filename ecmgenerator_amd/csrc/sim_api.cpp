@@ -1,0 +1,1169 @@
+// sim_api.cpp -- host runtime behind include/ecm_sim.h: device buffers,
+// static spatial indices (obstacles, ECM cells), the slot pool and the step
+// loop.  Compiled by hipcc for gfx950; the kernels live in sim_kernels.hip.
+#include "../../include/ecm_sim.h"
+#include "ecm_geom.h"
+#include "sim_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace ecm {
+hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
+hipError_t launch_attr_knn(const DevState& s, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st);
+hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
+hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
+                                const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
+                                float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
+                                int* path_cnt, hipStream_t st);
+}  // namespace ecm
+
+using namespace ecm;
+
+static thread_local std::string g_err;
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) return fail(ECM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char* ecm_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ecm_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
+    return std::strncmp(pr.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return fail(ECM_E_HIP, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        }
+        n = count;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+static GridDesc make_grid(float x0, float y0, float x1, float y1, float cell, int max_side) {
+    GridDesc g;
+    float ext = std::max(x1 - x0, y1 - y0);
+    if (cell * max_side < ext) cell = ext / max_side;
+    g.x0 = x0;
+    g.y0 = y0;
+    g.cell = cell;
+    g.inv_cell = 1.0f / cell;
+    g.side = std::max(1, (int)std::ceil(ext / cell));
+    int bits = 0;
+    while ((1 << bits) < g.side) ++bits;
+    g.bits = bits;
+    return g;
+}
+
+struct ecm_sim {
+    ecm_sim_params prm{};
+    float wx0 = 0, wy0 = 0, wx1 = 0, wy1 = 0;
+    hipStream_t stream = nullptr;
+    int dev = 0;
+
+    // agent SoA
+    DBuf<float2> pos[2], vel[2], attraction, pref_vel, v_orca;
+    DBuf<float> radius, pref_speed;
+    DBuf<uint8_t> active;
+    DBuf<int> path_off, path_cnt;
+    DBuf<float2> path_pool;
+    size_t path_used = 0;
+    int cur = 0;
+
+    // agent grid
+    GridDesc ag{};
+    int grid_for_n = -1;
+    DBuf<uint32_t> key, cell_count, cell_start, bsum;
+    DBuf<float4> sorted_pr;
+    DBuf<float2> sorted_v;
+    DBuf<int> nbr, nbr_cnt;
+
+    // events (device) + pinned mirror of the counters
+    DBuf<int> destroyed_list, replan_list;
+    DBuf<uint32_t> counters;     // [0] destroyed_n [1] replan_n [2..5] stats
+    uint32_t* h_counters = nullptr;
+
+    // obstacles
+    std::vector<float> obs_xy;
+    std::vector<int> obs_ring_off;
+    int n_obst_edges = 0;
+    DBuf<float4> oe0, oe1;
+    DBuf<uint8_t> oflags;
+    GridDesc og{};
+    DBuf<int> obucket_start, obucket_edges;
+    int maxo = 16;
+    float obst_range_built = -1.0f;
+
+    // ECM
+    std::vector<float> ecm_vert;
+    std::vector<int> ecm_target;
+    std::vector<float> ecm_pts;
+    int n_cells = 0;
+    DBuf<float4> cellA, cellB;
+    GridDesc eg{};
+    DBuf<int> ebucket_start, ebucket_cells;
+    int yhash_mask = -1;
+    DBuf<uint32_t> yhash_key;
+    DBuf<int> yhash_off, yhash_cnt, ycells;
+    bool ecm_set = false;
+
+    // host mirrors of the slot pool
+    std::vector<int> free_stack;         // top = back()
+    std::vector<uint8_t> h_active;
+    std::vector<float> h_radius, h_speed;
+    std::vector<float2> h_goal;          // last path point per slot
+    std::vector<int> h_path_cnt;
+    int num_agents = 0;
+    int last_idx = -1;
+
+    // callbacks, stats, timing
+    ecm_replan_fn replan_fn = nullptr;
+    void* replan_user = nullptr;
+    uint64_t stat[6] = {0, 0, 0, 0, 0, 0};
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float last_ms[3] = {0, 0, 0};
+    std::vector<int> h_replans;
+};
+
+extern "C" void ecm_sim_default_params(ecm_sim_params* p) {
+    if (!p) return;
+    p->max_agents = 10000;
+    p->k = 10;
+    p->time_horizon = 2.0f;
+    p->time_horizon_obst = 2.0f;
+    p->max_speed = 2.0f;
+    p->max_speed_from_pref = 0;
+    p->dt = 0.1f;
+    p->mass = 0.8f;
+    p->arrival_radius = 20.0f;
+    p->delete_radius = 2.0f;
+    p->grid_cell = 0.0f;
+    p->keep_orca_velocity = 0;
+}
+
+extern "C" void ecm_sim_reference_params(ecm_sim_params* p) {
+    ecm_sim_default_params(p);
+    if (!p) return;
+    p->k = 5;                       // Simulator.cpp:55
+    p->time_horizon = 10.0f;        // ORCA.h:102
+    p->time_horizon_obst = 10.0f;   // ORCA.h:103
+    p->max_speed_from_pref = 1;     // Simulator.cpp:671-674
+}
+
+static void free_all(ecm_sim* s) {
+    for (int b = 0; b < 2; ++b) { s->pos[b].release(); s->vel[b].release(); }
+    s->attraction.release(); s->pref_vel.release(); s->v_orca.release();
+    s->radius.release(); s->pref_speed.release(); s->active.release();
+    s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
+    s->key.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
+    s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
+    s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
+    s->oe0.release(); s->oe1.release(); s->oflags.release();
+    s->obucket_start.release(); s->obucket_edges.release();
+    s->cellA.release(); s->cellB.release(); s->ebucket_start.release(); s->ebucket_cells.release();
+    s->yhash_key.release(); s->yhash_off.release(); s->yhash_cnt.release(); s->ycells.release();
+    if (s->h_counters) (void)hipHostFree(s->h_counters);
+    s->h_counters = nullptr;
+    for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->stream = nullptr;
+}
+
+extern "C" void ecm_sim_destroy(ecm_sim* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->dev);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+}
+
+static int zero_dev(void* p, size_t bytes, hipStream_t st) {
+    HIPCHK(hipMemsetAsync(p, 0, bytes, st));
+    return 0;
+}
+
+extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float x1, float y1, ecm_sim** out) {
+    if (!p || !out) return fail(ECM_E_ARG, "null argument");
+    *out = nullptr;
+    if (p->max_agents <= 0 || p->k < 1 || p->k > 16) return fail(ECM_E_ARG, "max_agents > 0 and 1 <= k <= 16 required");
+    if (!(x1 > x0) || !(y1 > y0)) return fail(ECM_E_ARG, "empty world box");
+    if (!(p->dt > 0) || !(p->time_horizon > 0) || !(p->time_horizon_obst > 0) || !(p->mass > 0))
+        return fail(ECM_E_ARG, "dt, time horizons and mass must be positive");
+    if (!ecm_device_available()) return fail(ECM_E_NOGPU, "no gfx950 device available");
+    ecm_sim* s = new ecm_sim();
+    s->prm = *p;
+    s->wx0 = x0; s->wy0 = y0; s->wx1 = x1; s->wy1 = y1;
+    const size_t N = (size_t)p->max_agents;
+    int rc = 0;
+    if (hipGetDevice(&s->dev) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(ECM_E_HIP, "stream creation failed");
+    }
+    for (int b = 0; b < 2 && !rc; ++b) { rc |= s->pos[b].alloc(N); rc |= s->vel[b].alloc(N); }
+    if (!rc) rc |= s->attraction.alloc(N);
+    if (!rc) rc |= s->pref_vel.alloc(N);
+    if (!rc && p->keep_orca_velocity) rc |= s->v_orca.alloc(N);
+    if (!rc) rc |= s->radius.alloc(N);
+    if (!rc) rc |= s->pref_speed.alloc(N);
+    if (!rc) rc |= s->active.alloc(N);
+    if (!rc) rc |= s->path_off.alloc(N);
+    if (!rc) rc |= s->path_cnt.alloc(N);
+    if (!rc) rc |= s->path_pool.alloc(std::max<size_t>(N * 8, 1024));
+    if (!rc) rc |= s->key.alloc(N);
+    if (!rc) rc |= s->sorted_pr.alloc(N);
+    if (!rc) rc |= s->sorted_v.alloc(N);
+    if (!rc) rc |= s->nbr.alloc(N * (size_t)p->k);
+    if (!rc) rc |= s->nbr_cnt.alloc(N);
+    if (!rc) rc |= s->destroyed_list.alloc(N);
+    if (!rc) rc |= s->replan_list.alloc(N);
+    if (!rc) rc |= s->counters.alloc(8);
+    if (!rc && hipHostMalloc((void**)&s->h_counters, 8 * sizeof(uint32_t)) != hipSuccess)
+        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
+    for (int i = 0; i < 4 && !rc; ++i)
+        if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
+    if (!rc) {
+        for (int b = 0; b < 2 && !rc; ++b) {
+            rc |= zero_dev(s->pos[b].p, N * sizeof(float2), s->stream);
+            rc |= zero_dev(s->vel[b].p, N * sizeof(float2), s->stream);
+        }
+        rc |= zero_dev(s->attraction.p, N * sizeof(float2), s->stream);
+        rc |= zero_dev(s->pref_vel.p, N * sizeof(float2), s->stream);
+        rc |= zero_dev(s->active.p, N, s->stream);
+        rc |= zero_dev(s->path_cnt.p, N * sizeof(int), s->stream);
+        rc |= zero_dev(s->path_off.p, N * sizeof(int), s->stream);
+        rc |= zero_dev(s->radius.p, N * sizeof(float), s->stream);
+        rc |= zero_dev(s->pref_speed.p, N * sizeof(float), s->stream);
+        rc |= zero_dev(s->counters.p, 8 * sizeof(uint32_t), s->stream);
+        if (s->v_orca.p) rc |= zero_dev(s->v_orca.p, N * sizeof(float2), s->stream);
+        if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(ECM_E_HIP, "sync failed");
+    }
+    if (rc) { free_all(s); delete s; return rc; }
+    // LIFO pool, top = slot 0 (Simulator.h:66-69)
+    s->free_stack.reserve(N);
+    for (int i = (int)N - 1; i >= 0; --i) s->free_stack.push_back(i);
+    s->h_active.assign(N, 0);
+    s->h_radius.assign(N, 0.0f);
+    s->h_speed.assign(N, 0.0f);
+    s->h_goal.assign(N, make_float2(0.0f, 0.0f));
+    s->h_path_cnt.assign(N, 0);
+    *out = s;
+    return 0;
+}
+
+// ------------------------------------------------------------- obstacles
+static float obstacle_range(const ecm_sim* s) {
+    float rmax = 0.0f, vmax = s->prm.max_speed;
+    for (size_t i = 0; i < s->h_active.size(); ++i) {
+        if (!s->h_active[i]) continue;
+        rmax = std::max(rmax, s->h_radius[i]);
+        if (s->prm.max_speed_from_pref) vmax = std::max(vmax, s->h_speed[i]);
+    }
+    if (s->prm.max_speed_from_pref && s->num_agents == 0) vmax = 0.0f;
+    return s->prm.time_horizon_obst * vmax + rmax;
+}
+
+// squared distance from segment ab to the box [bx0,bx1]x[by0,by1] (double)
+static double seg_box_dist2(double ax, double ay, double bx, double by, double x0, double y0, double x1, double y1) {
+    auto inside = [&](double x, double y) { return x >= x0 && x <= x1 && y >= y0 && y <= y1; };
+    if (inside(ax, ay) || inside(bx, by)) return 0.0;
+    auto pt_seg = [](double px, double py, double sx, double sy, double tx, double ty) {
+        double dx = tx - sx, dy = ty - sy, l2 = dx * dx + dy * dy;
+        double u = l2 > 0 ? ((px - sx) * dx + (py - sy) * dy) / l2 : 0.0;
+        u = std::min(1.0, std::max(0.0, u));
+        double qx = sx + u * dx - px, qy = sy + u * dy - py;
+        return qx * qx + qy * qy;
+    };
+    auto segs_cross = [](double ax, double ay, double bx, double by, double cx, double cy, double dx, double dy) {
+        auto orient = [](double px, double py, double qx, double qy, double rx, double ry) {
+            double v = (qx - px) * (ry - py) - (qy - py) * (rx - px);
+            return (v > 0) - (v < 0);
+        };
+        int o1 = orient(ax, ay, bx, by, cx, cy), o2 = orient(ax, ay, bx, by, dx, dy);
+        int o3 = orient(cx, cy, dx, dy, ax, ay), o4 = orient(cx, cy, dx, dy, bx, by);
+        return o1 * o2 <= 0 && o3 * o4 <= 0;
+    };
+    double ex[4][4] = {{x0, y0, x1, y0}, {x1, y0, x1, y1}, {x1, y1, x0, y1}, {x0, y1, x0, y0}};
+    double best = 1e300;
+    for (auto& e : ex) {
+        if (segs_cross(ax, ay, bx, by, e[0], e[1], e[2], e[3])) return 0.0;
+        best = std::min(best, pt_seg(e[0], e[1], ax, ay, bx, by));
+        best = std::min(best, pt_seg(e[2], e[3], ax, ay, bx, by));
+        best = std::min(best, pt_seg(ax, ay, e[0], e[1], e[2], e[3]));
+        best = std::min(best, pt_seg(bx, by, e[0], e[1], e[2], e[3]));
+    }
+    return best;
+}
+
+// Bucket every obstacle edge into the cells within `range` of it (capsule
+// test), lists ascending by edge id = (obstacle, vertex) scan order.
+static int build_obstacle_buckets(ecm_sim* s, float range) {
+    const int ne = s->n_obst_edges;
+    const float margin = 0.05f;
+    const double R = (double)range + margin;
+    float cell = 2.0f;
+    for (int attempt = 0; attempt < 6; ++attempt, cell *= 0.5f) {
+        GridDesc g = make_grid(s->wx0, s->wy0, s->wx1, s->wy1, cell, 4096);
+        const int side = g.side;
+        std::vector<std::vector<int>> lists((size_t)side * side);
+        // edge endpoints from the host copy
+        const std::vector<int>& ro = s->obs_ring_off;
+        int nr = (int)ro.size() - 1;
+        for (int r = 0; r < nr; ++r) {
+            int b = ro[r], m = ro[r + 1] - b;
+            for (int i = 0; i < m; ++i) {
+                int e = b + i, f = b + (i + 1) % m;
+                double ax = s->obs_xy[2 * e], ay = s->obs_xy[2 * e + 1];
+                double bx = s->obs_xy[2 * f], by = s->obs_xy[2 * f + 1];
+                double lx = std::min(ax, bx) - R, hx = std::max(ax, bx) + R;
+                double ly = std::min(ay, by) - R, hy = std::max(ay, by) + R;
+                int cx0 = grid_cell((float)lx, g.x0, g.inv_cell, side), cx1 = grid_cell((float)hx, g.x0, g.inv_cell, side);
+                int cy0 = grid_cell((float)ly, g.y0, g.inv_cell, side), cy1 = grid_cell((float)hy, g.y0, g.inv_cell, side);
+                for (int cy = cy0; cy <= cy1; ++cy)
+                    for (int cx = cx0; cx <= cx1; ++cx) {
+                        double bx0 = g.x0 + (double)cx * g.cell, by0 = g.y0 + (double)cy * g.cell;
+                        double bx1 = bx0 + g.cell, by1 = by0 + g.cell;
+                        // cells on the grid border also hold points beyond the box
+                        if (cx == 0) bx0 = -1e30;
+                        if (cy == 0) by0 = -1e30;
+                        if (cx == side - 1) bx1 = 1e30;
+                        if (cy == side - 1) by1 = 1e30;
+                        if (seg_box_dist2(ax, ay, bx, by, bx0, by0, bx1, by1) <= R * R)
+                            lists[(size_t)cy * side + cx].push_back(e);
+                    }
+            }
+        }
+        size_t maxlen = 0, total = 0;
+        for (auto& l : lists) { maxlen = std::max(maxlen, l.size()); total += l.size(); }
+        if (maxlen > 64 && attempt < 5) continue;
+        if (maxlen > 64) return fail(ECM_E_STATE, "more than 64 obstacle edges within range of one point");
+        std::vector<int> start((size_t)side * side + 1), edges;
+        edges.reserve(total);
+        for (size_t c = 0; c < lists.size(); ++c) {
+            start[c] = (int)edges.size();
+            edges.insert(edges.end(), lists[c].begin(), lists[c].end());
+        }
+        start[lists.size()] = (int)edges.size();
+        int rc = s->obucket_start.alloc(start.size());
+        if (!rc) rc = s->obucket_edges.alloc(std::max<size_t>(edges.size(), 1));
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(s->obucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        if (!edges.empty())
+            HIPCHK(hipMemcpyAsync(s->obucket_edges.p, edges.data(), edges.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->og = g;
+        s->maxo = maxlen <= 16 ? 16 : (maxlen <= 32 ? 32 : 64);
+        s->obst_range_built = range;
+        return 0;
+    }
+    return fail(ECM_E_STATE, "obstacle bucketing failed");
+}
+
+extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* ring_off, int n_rings) {
+    if (!s || n_rings < 0 || (n_rings > 0 && (!xy || !ring_off))) return fail(ECM_E_ARG, "bad obstacle arguments");
+    (void)hipSetDevice(s->dev);
+    std::vector<int> ro(ring_off, ring_off + n_rings + 1);
+    if (n_rings == 0) ro.assign(1, 0);
+    if (ro[0] != 0) return fail(ECM_E_ARG, "ring_off[0] must be 0");
+    for (int r = 0; r < n_rings; ++r)
+        if (ro[r + 1] - ro[r] < 2) return fail(ECM_E_ARG, "obstacle %d has fewer than 2 vertices (Environment.cpp:288)", r);
+    const int nv = ro[n_rings];
+    s->obs_xy.assign(xy, xy + 2 * (size_t)nv);
+    s->obs_ring_off = ro;
+    s->n_obst_edges = nv;
+    if (nv == 0) { s->obst_range_built = -1.0f; return 0; }
+    // Obstacle::Initialize (ECMDataTypes.cpp:23-61): prev/next in the ring,
+    // isConvex = det(prev - next, p - prev) >= 0 (true for 2-vertex rings).
+    std::vector<float4> e0(nv), e1(nv);
+    std::vector<uint8_t> fl(nv);
+    std::vector<uint8_t> convex(nv);
+    std::vector<int> nxt(nv), prv(nv);
+    for (int r = 0; r < n_rings; ++r) {
+        int b = ro[r], m = ro[r + 1] - b;
+        for (int i = 0; i < m; ++i) {
+            int v = b + i;
+            prv[v] = (i == 0) ? b + m - 1 : b + i - 1;
+            nxt[v] = (i == 0) ? b + 1 : b + (i + 1) % m;
+        }
+        for (int i = 0; i < m; ++i) {
+            int v = b + i;
+            if (m == 2) { convex[v] = 1; continue; }
+            V2 P = mk(xy[2 * v], xy[2 * v + 1]);
+            V2 A = mk(xy[2 * prv[v]], xy[2 * prv[v] + 1]);
+            V2 B = mk(xy[2 * nxt[v]], xy[2 * nxt[v] + 1]);
+            convex[v] = det(A - B, P - A) >= 0.0f;
+        }
+    }
+    for (int v = 0; v < nv; ++v) {
+        int R = nxt[v];
+        e0[v] = make_float4(xy[2 * v], xy[2 * v + 1], xy[2 * R], xy[2 * R + 1]);
+        e1[v] = make_float4(xy[2 * prv[v]], xy[2 * prv[v] + 1], xy[2 * nxt[R]], xy[2 * nxt[R] + 1]);
+        fl[v] = (uint8_t)(convex[v] | (convex[R] << 1));
+    }
+    int rc = s->oe0.alloc(nv);
+    if (!rc) rc = s->oe1.alloc(nv);
+    if (!rc) rc = s->oflags.alloc(nv);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->oe0.p, e0.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->oe1.p, e1.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->oflags.p, fl.data(), nv, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->obst_range_built = -1.0f;   // (re)bucketed at the next step
+    return 0;
+}
+
+// ------------------------------------------------------------------- ECM
+static uint32_t hash_u32_host(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int n_edge, const int* he_target,
+                               const float* he_pts) {
+    if (!s || n_vert < 0 || n_edge < 0 || (n_edge > 0 && (!vert_xy || !he_target || !he_pts)))
+        return fail(ECM_E_ARG, "bad ECM arguments");
+    (void)hipSetDevice(s->dev);
+    for (int e = 0; e < 2 * n_edge; ++e)
+        if (he_target[e] < 0 || he_target[e] >= n_vert) return fail(ECM_E_ARG, "half-edge target out of range");
+    s->ecm_vert.assign(vert_xy, vert_xy + 2 * (size_t)n_vert);
+    s->ecm_target.assign(he_target, he_target + 2 * (size_t)n_edge);
+    s->ecm_pts.assign(he_pts, he_pts + 8 * (size_t)n_edge);
+    const int nc = 2 * n_edge;
+    s->n_cells = nc;
+    s->ecm_set = true;
+    if (nc == 0) { s->yhash_mask = -1; return 0; }
+    // cells: left (he0.cl, he1.cr), right (he0.cr, he1.cl); ring
+    // (v(he1.target), b.p0, b.p1, v(he0.target)) (ECMCellCollection.cpp:18-46, 57-90)
+    std::vector<float4> A(nc), B(nc);
+    for (int e = 0; e < n_edge; ++e) {
+        const float* v0 = &vert_xy[2 * he_target[2 * e + 1]];
+        const float* v1 = &vert_xy[2 * he_target[2 * e + 0]];
+        const float* h = &he_pts[8 * e];
+        A[2 * e] = make_float4(v0[0], v0[1], h[0], h[1]);
+        B[2 * e] = make_float4(h[6], h[7], v1[0], v1[1]);
+        A[2 * e + 1] = make_float4(v0[0], v0[1], h[2], h[3]);
+        B[2 * e + 1] = make_float4(h[4], h[5], v1[0], v1[1]);
+    }
+    // bucket grid: mean cell box area decides the bucket edge
+    double area = 0.0;
+    float bx0 = s->wx0, by0 = s->wy0, bx1 = s->wx1, by1 = s->wy1;
+    std::vector<float> box(4 * (size_t)nc);
+    const float margin = 0.01f;
+    for (int c = 0; c < nc; ++c) {
+        float xs[4] = {A[c].x, A[c].z, B[c].x, B[c].z}, ys[4] = {A[c].y, A[c].w, B[c].y, B[c].w};
+        float lx = *std::min_element(xs, xs + 4), hx = *std::max_element(xs, xs + 4);
+        float ly = *std::min_element(ys, ys + 4), hy = *std::max_element(ys, ys + 4);
+        box[4 * c] = lx - margin; box[4 * c + 1] = ly - margin; box[4 * c + 2] = hx + margin; box[4 * c + 3] = hy + margin;
+        area += (double)(hx - lx) * (hy - ly);
+        bx0 = std::min(bx0, lx); by0 = std::min(by0, ly); bx1 = std::max(bx1, hx); by1 = std::max(by1, hy);
+    }
+    float cell = (float)std::sqrt(std::max(area / nc, 1e-4));
+    cell = std::min(std::max(cell, 1.0f), 16.0f);
+    GridDesc g = make_grid(bx0 - 1.0f, by0 - 1.0f, bx1 + 1.0f, by1 + 1.0f, cell, 4096);
+    const int side = g.side;
+    std::vector<int> cnt((size_t)side * side + 1, 0);
+    auto range_of = [&](int c, int& cx0, int& cx1, int& cy0, int& cy1) {
+        cx0 = grid_cell(box[4 * c], g.x0, g.inv_cell, side);
+        cy0 = grid_cell(box[4 * c + 1], g.y0, g.inv_cell, side);
+        cx1 = grid_cell(box[4 * c + 2], g.x0, g.inv_cell, side);
+        cy1 = grid_cell(box[4 * c + 3], g.y0, g.inv_cell, side);
+    };
+    for (int c = 0; c < nc; ++c) {
+        int cx0, cx1, cy0, cy1;
+        range_of(c, cx0, cx1, cy0, cy1);
+        for (int y = cy0; y <= cy1; ++y)
+            for (int x = cx0; x <= cx1; ++x) cnt[(size_t)y * side + x]++;
+    }
+    std::vector<int> start((size_t)side * side + 1, 0);
+    for (size_t i = 0; i < (size_t)side * side; ++i) start[i + 1] = start[i] + cnt[i];
+    std::vector<int> cells(start.back());
+    std::vector<int> fillp(start.begin(), start.end() - 1);
+    for (int c = 0; c < nc; ++c) {            // ascending c -> ascending lists
+        int cx0, cx1, cy0, cy1;
+        range_of(c, cx0, cx1, cy0, cy1);
+        for (int y = cy0; y <= cy1; ++y)
+            for (int x = cx0; x <= cx1; ++x) cells[fillp[(size_t)y * side + x]++] = c;
+    }
+    // exact-y lists: every distinct vertex y of every cell
+    std::vector<std::pair<uint32_t, int>> yc;
+    yc.reserve(4 * (size_t)nc);
+    for (int c = 0; c < nc; ++c) {
+        float ys[4] = {A[c].y, A[c].w, B[c].y, B[c].w};
+        for (float y : ys) {
+            uint32_t k;
+            float yy = (y == 0.0f) ? 0.0f : y;
+            std::memcpy(&k, &yy, 4);
+            if (y != y) continue;
+            yc.push_back({k, c});
+        }
+    }
+    std::sort(yc.begin(), yc.end());
+    yc.erase(std::unique(yc.begin(), yc.end()), yc.end());
+    std::vector<uint32_t> keys;
+    std::vector<int> koff, kcnt, ycells;
+    for (size_t i = 0; i < yc.size();) {
+        size_t j = i;
+        keys.push_back(yc[i].first);
+        koff.push_back((int)ycells.size());
+        while (j < yc.size() && yc[j].first == yc[i].first) ycells.push_back(yc[j++].second);
+        kcnt.push_back((int)(j - i));
+        i = j;
+    }
+    size_t tsize = 1;
+    while (tsize < 2 * keys.size() + 2) tsize <<= 1;
+    std::vector<uint32_t> tkey(tsize, 0xffffffffu);
+    std::vector<int> toff(tsize, 0), tcnt(tsize, 0);
+    for (size_t i = 0; i < keys.size(); ++i) {
+        uint32_t h = hash_u32_host(keys[i]) & (uint32_t)(tsize - 1);
+        while (tkey[h] != 0xffffffffu) h = (h + 1) & (uint32_t)(tsize - 1);
+        tkey[h] = keys[i]; toff[h] = koff[i]; tcnt[h] = kcnt[i];
+    }
+    int rc = s->cellA.alloc(nc);
+    if (!rc) rc = s->cellB.alloc(nc);
+    if (!rc) rc = s->ebucket_start.alloc(start.size());
+    if (!rc) rc = s->ebucket_cells.alloc(cells.size());
+    if (!rc) rc = s->yhash_key.alloc(tsize);
+    if (!rc) rc = s->yhash_off.alloc(tsize);
+    if (!rc) rc = s->yhash_cnt.alloc(tsize);
+    if (!rc) rc = s->ycells.alloc(ycells.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->cellA.p, A.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->cellB.p, B.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->ebucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    if (!cells.empty())
+        HIPCHK(hipMemcpyAsync(s->ebucket_cells.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->yhash_key.p, tkey.data(), tsize * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->yhash_off.p, toff.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->yhash_cnt.p, tcnt.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    if (!ycells.empty())
+        HIPCHK(hipMemcpyAsync(s->ycells.p, ycells.data(), ycells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->eg = g;
+    s->yhash_mask = (int)tsize - 1;
+    return 0;
+}
+
+// ------------------------------------------------------------- agents
+static int ensure_path_capacity(ecm_sim* s, size_t extra) {
+    if (s->path_used + extra <= s->path_pool.n) return 0;
+    size_t cap = std::max(s->path_pool.n * 2, s->path_used + extra + 1024);
+    float2* np = nullptr;
+    if (hipMalloc((void**)&np, cap * sizeof(float2)) != hipSuccess) return fail(ECM_E_HIP, "path pool growth failed");
+    if (s->path_used)
+        HIPCHK(hipMemcpyAsync(np, s->path_pool.p, s->path_used * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->path_pool.release();
+    s->path_pool.p = np;
+    s->path_pool.n = cap;
+    return 0;
+}
+
+struct HostGrid {
+    float x0, y0, inv;
+    int side;
+    std::vector<std::vector<int>> cells;
+    void init(float ax0, float ay0, float ax1, float ay1, float cell) {
+        x0 = ax0; y0 = ay0; inv = 1.0f / cell;
+        side = std::max(1, std::min(4096, (int)std::ceil(std::max(ax1 - ax0, ay1 - ay0) / cell)));
+        cells.assign((size_t)side * side, {});
+    }
+    int cx(float x) const { return grid_cell(x, x0, inv, side); }
+    int cy(float y) const { return grid_cell(y, y0, inv, side); }
+    void add(int id, float x, float y) { cells[(size_t)cy(y) * side + cx(x)].push_back(id); }
+};
+
+extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const float* radius, const float* pref_speed,
+                             const int* path_off, const int* path_cnt, const float* path_xy, int validate,
+                             int* out_ids) {
+    if (!s || n < 0 || (n > 0 && (!start_xy || !radius || !pref_speed || !path_off || !path_cnt || !path_xy || !out_ids)))
+        return fail(ECM_E_ARG, "bad spawn arguments");
+    (void)hipSetDevice(s->dev);
+    if (n == 0) return 0;
+    for (int i = 0; i < n; ++i)
+        if (path_cnt[i] < 1 || path_off[i] < 0)
+            return fail(ECM_E_ARG, "agent %d: empty path (the reference reads path.x[-1], Simulator.cpp:554)", i);
+    // ValidSpawnLocation (Simulator.cpp:295-311) against active agents,
+    // including those spawned earlier in this batch.
+    HostGrid hg;
+    float rmax_new = 0.0f;
+    for (int i = 0; i < n; ++i) rmax_new = std::max(rmax_new, radius[i]);
+    const float gcell = std::max(1.0f, rmax_new);
+    std::vector<float2> cur_pos;
+    if (validate) {
+        hg.init(s->wx0 - 1, s->wy0 - 1, s->wx1 + 1, s->wy1 + 1, gcell);
+        if (s->num_agents > 0) {
+            cur_pos.resize(s->prm.max_agents);
+            HIPCHK(hipMemcpyAsync(cur_pos.data(), s->pos[s->cur].p, cur_pos.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+            for (int i = 0; i <= s->last_idx; ++i)
+                if (s->h_active[i]) hg.add(i, cur_pos[i].x, cur_pos[i].y);
+        }
+    }
+    std::vector<float2> placed(n);
+    std::vector<int> ids(n, -1);
+    std::vector<int> offs(n);
+    size_t total_pts = 0;
+    for (int i = 0; i < n; ++i) total_pts += (size_t)path_cnt[i];
+    int rc = ensure_path_capacity(s, total_pts);
+    if (rc) return rc;
+    std::vector<float2> pts;
+    pts.reserve(total_pts);
+    for (int i = 0; i < n; ++i) {
+        out_ids[i] = -1;
+        if (s->free_stack.empty()) continue;                       // Simulator.cpp:170
+        float x = start_xy[2 * i], y = start_xy[2 * i + 1];
+        if (validate) {
+            const float c2 = radius[i] * radius[i];
+            bool ok = true;
+            int gx = hg.cx(x), gy = hg.cy(y);
+            int reach = (int)std::ceil(radius[i] * hg.inv) + 1;
+            for (int yy = std::max(0, gy - reach); ok && yy <= std::min(hg.side - 1, gy + reach); ++yy)
+                for (int xx = std::max(0, gx - reach); ok && xx <= std::min(hg.side - 1, gx + reach); ++xx)
+                    for (int j : hg.cells[(size_t)yy * hg.side + xx]) {
+                        float px = (j >= 0) ? cur_pos[j].x : placed[-j - 1].x;
+                        float py = (j >= 0) ? cur_pos[j].y : placed[-j - 1].y;
+                        float dx = x - px, dy = y - py;
+                        if (dx * dx + dy * dy < c2) { ok = false; break; }
+                    }
+            if (!ok) continue;                                     // Simulator.cpp:172
+        }
+        int id = s->free_stack.back();
+        s->free_stack.pop_back();
+        ++s->num_agents;
+        s->last_idx = std::max(s->last_idx, id);
+        ids[i] = id;
+        out_ids[i] = id;
+        placed[i] = make_float2(x, y);
+        if (validate) hg.add(-i - 1, x, y);
+        s->h_active[id] = 1;
+        s->h_radius[id] = radius[i];
+        s->h_speed[id] = pref_speed[i];
+        int c = path_cnt[i];
+        offs[i] = (int)(s->path_used + pts.size());
+        for (int q = 0; q < c; ++q) pts.push_back(make_float2(path_xy[2 * ((size_t)path_off[i] + q)], path_xy[2 * ((size_t)path_off[i] + q) + 1]));
+        s->h_goal[id] = pts.back();
+        s->h_path_cnt[id] = c;
+    }
+    if (!pts.empty())
+        HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, pts.data(), pts.size() * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    s->path_used += pts.size();
+    // upload the batch and scatter it into the SoA
+    std::vector<float2> st(n);
+    for (int i = 0; i < n; ++i) st[i] = make_float2(start_xy[2 * i], start_xy[2 * i + 1]);
+    DBuf<int> d_ids, d_off, d_cnt;
+    DBuf<float2> d_st;
+    DBuf<float> d_r, d_s;
+    rc = d_ids.alloc(n);
+    if (!rc) rc = d_off.alloc(n);
+    if (!rc) rc = d_cnt.alloc(n);
+    if (!rc) rc = d_st.alloc(n);
+    if (!rc) rc = d_r.alloc(n);
+    if (!rc) rc = d_s.alloc(n);
+    if (!rc) {
+        HIPCHK(hipMemcpyAsync(d_ids.p, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d_off.p, offs.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d_cnt.p, path_cnt, n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d_st.p, st.data(), n * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d_r.p, radius, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d_s.p, pref_speed, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(launch_spawn_scatter(n, d_ids.p, d_st.p, d_r.p, d_s.p, d_off.p, d_cnt.p, s->pos[s->cur].p,
+                                    s->vel[s->cur].p, s->radius.p, s->pref_speed.p, s->pref_vel.p, s->active.p,
+                                    s->path_off.p, s->path_cnt.p, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    d_ids.release(); d_off.release(); d_cnt.release(); d_st.release(); d_r.release(); d_s.release();
+    return rc;
+}
+
+static void mark_destroyed(ecm_sim* s, int id) {
+    // DestroyAgent (Simulator.cpp:202-208)
+    --s->num_agents;
+    s->h_active[id] = 0;
+    s->free_stack.push_back(id);
+}
+
+extern "C" int ecm_sim_destroy_agent(ecm_sim* s, int id) {
+    if (!s || id < 0 || id >= s->prm.max_agents) return fail(ECM_E_ARG, "bad agent id");
+    if (!s->h_active[id]) return fail(ECM_E_ARG, "agent %d is not active", id);
+    (void)hipSetDevice(s->dev);
+    uint8_t z = 0;
+    HIPCHK(hipMemcpyAsync(s->active.p + id, &z, 1, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    mark_destroyed(s, id);
+    return 0;
+}
+
+extern "C" int ecm_sim_reset(ecm_sim* s) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    for (int i = 0; i <= s->last_idx; ++i)        // Simulator.cpp:325-333
+        if (s->h_active[i]) {
+            int rc = ecm_sim_destroy_agent(s, i);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+extern "C" int ecm_sim_set_path(ecm_sim* s, int id, const float* xy, int n) {
+    if (!s || id < 0 || id >= s->prm.max_agents || n < 1 || !xy) return fail(ECM_E_ARG, "bad set_path arguments");
+    (void)hipSetDevice(s->dev);
+    int rc = ensure_path_capacity(s, (size_t)n);
+    if (rc) return rc;
+    int off = (int)s->path_used;
+    HIPCHK(hipMemcpyAsync(s->path_pool.p + off, xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->path_off.p + id, &off, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->path_cnt.p + id, &n, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->path_used += (size_t)n;
+    s->h_goal[id] = make_float2(xy[2 * (n - 1)], xy[2 * (n - 1) + 1]);
+    s->h_path_cnt[id] = n;
+    return 0;
+}
+
+extern "C" int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    s->replan_fn = fn;
+    s->replan_user = user;
+    return 0;
+}
+
+// ------------------------------------------------------------------ step
+static int ensure_agent_grid(ecm_sim* s) {
+    int n = std::max(1, s->num_agents);
+    if (s->grid_for_n > 0 && n <= 2 * s->grid_for_n && 2 * n >= s->grid_for_n && s->prm.grid_cell <= 0.0f) return 0;
+    if (s->grid_for_n > 0 && s->prm.grid_cell > 0.0f) return 0;
+    float ext = std::max(s->wx1 - s->wx0, s->wy1 - s->wy0);
+    float cell = s->prm.grid_cell;
+    if (cell <= 0.0f) {
+        double area = (double)(s->wx1 - s->wx0) * (s->wy1 - s->wy0);
+        cell = (float)std::sqrt(area / n) * 1.0f;
+        cell = std::min(std::max(cell, 0.5f), 32.0f);
+    }
+    GridDesc g = make_grid(s->wx0, s->wy0, s->wx1, s->wy1, cell, 2048);
+    (void)ext;
+    const size_t G = (size_t)1 << (2 * g.bits);
+    int rc = s->cell_count.alloc(G);
+    if (!rc) rc = s->cell_start.alloc(G + 1);
+    if (!rc) rc = s->bsum.alloc((G + 1023) / 1024 + 1);
+    if (rc) return rc;
+    rc = zero_dev(s->cell_count.p, G * sizeof(uint32_t), s->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->ag = g;
+    s->grid_for_n = n;
+    return 0;
+}
+
+static DevState make_state(ecm_sim* s) {
+    DevState d;
+    std::memset(&d, 0, sizeof(d));
+    d.n_slots = s->prm.max_agents;
+    d.pos_in = s->pos[s->cur].p; d.pos_out = s->pos[s->cur ^ 1].p;
+    d.vel_in = s->vel[s->cur].p; d.vel_out = s->vel[s->cur ^ 1].p;
+    d.radius = s->radius.p; d.pref_speed = s->pref_speed.p;
+    d.attraction = s->attraction.p; d.pref_vel = s->pref_vel.p; d.active = s->active.p;
+    d.path_off = s->path_off.p; d.path_cnt = s->path_cnt.p; d.path_pts = s->path_pool.p;
+    d.v_orca = s->v_orca.p;
+    d.ag = s->ag;
+    d.key = s->key.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
+    d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
+    d.destroyed_list = s->destroyed_list.p; d.destroyed_n = s->counters.p;
+    d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
+    d.stats = s->counters.p + 2;
+    d.n_obst_edges = s->n_obst_edges;
+    d.oe0 = s->oe0.p; d.oe1 = s->oe1.p; d.oflags = s->oflags.p; d.og = s->og;
+    d.obucket_start = s->obucket_start.p; d.obucket_edges = s->obucket_edges.p;
+    d.n_cells = s->n_cells; d.cellA = s->cellA.p; d.cellB = s->cellB.p; d.eg = s->eg;
+    d.ebucket_start = s->ebucket_start.p; d.ebucket_cells = s->ebucket_cells.p;
+    d.yhash_mask = s->yhash_mask; d.yhash_key = s->yhash_key.p; d.yhash_off = s->yhash_off.p;
+    d.yhash_cnt = s->yhash_cnt.p; d.ycells = s->ycells.p;
+    d.k = s->prm.k;
+    d.time_horizon = s->prm.time_horizon;
+    d.time_horizon_obst = s->prm.time_horizon_obst;
+    d.inv_time_horizon_obst = 1.0f / s->prm.time_horizon_obst;
+    d.max_speed = s->prm.max_speed;
+    d.max_speed_from_pref = s->prm.max_speed_from_pref;
+    d.dt = s->prm.dt;
+    d.mass_recip = 1.0f / s->prm.mass;
+    d.arrival_r2 = s->prm.arrival_radius * s->prm.arrival_radius;
+    d.delete_r2 = s->prm.delete_radius * s->prm.delete_radius;
+    return d;
+}
+
+static int step_once(ecm_sim* s) {
+    // UpdateMaxAgentIndex (Simulator.cpp:481-492)
+    while (s->last_idx >= 0 && !s->h_active[s->last_idx]) --s->last_idx;
+    int rc = ensure_agent_grid(s);
+    if (rc) return rc;
+    if (s->n_obst_edges > 0) {
+        float range = obstacle_range(s);
+        if (s->obst_range_built < range) {
+            rc = build_obstacle_buckets(s, range * 1.25f + 0.5f);
+            if (rc) return rc;
+        }
+    }
+    DevState d = make_state(s);
+    HIPCHK(launch_fill_u32(s->counters.p, 0u, 2, s->stream));
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
+    HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
+    HIPCHK(launch_attr_knn(d, s->stream));
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
+    HIPCHK(launch_orca(d, s->maxo, s->stream));
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->cur ^= 1;
+    const int n_before = s->num_agents;
+    uint32_t nd = s->h_counters[0], nr = s->h_counters[1];
+    if (nd) {
+        std::vector<int> ids(nd);
+        HIPCHK(hipMemcpy(ids.data(), s->destroyed_list.p, nd * sizeof(int), hipMemcpyDeviceToHost));
+        std::sort(ids.begin(), ids.end());      // reference visits slots in ascending order
+        for (int id : ids) mark_destroyed(s, id);
+    }
+    s->h_replans.clear();
+    if (nr) {
+        s->h_replans.resize(nr);
+        HIPCHK(hipMemcpy(s->h_replans.data(), s->replan_list.p, nr * sizeof(int), hipMemcpyDeviceToHost));
+        std::sort(s->h_replans.begin(), s->h_replans.end());
+        if (s->replan_fn) {
+            std::vector<float2> p(1);
+            std::vector<float> buf(2 * 4096);
+            for (int id : s->h_replans) {
+                HIPCHK(hipMemcpy(p.data(), s->pos[s->cur].p + id, sizeof(float2), hipMemcpyDeviceToHost));
+                int np = 0;
+                float2 g = s->h_goal[id];
+                int r = s->replan_fn(s->replan_user, id, p[0].x, p[0].y, g.x, g.y, s->h_radius[id], buf.data(), 4096, &np);
+                // UpdatePath (Simulator.cpp:97-124); an empty result keeps the old path
+                if (r == 0 && np >= 1) {
+                    rc = ecm_sim_set_path(s, id, buf.data(), np);
+                    if (rc) return rc;
+                }
+            }
+        }
+    }
+    s->stat[0] += 1;
+    s->stat[1] += (uint64_t)n_before;
+    s->stat[2] = s->h_counters[2];
+    s->stat[3] = s->h_counters[4];
+    s->stat[4] += nr;
+    s->stat[5] += nd;
+    if (s->timing) {
+        float a = 0, b = 0, c = 0;
+        HIPCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+        HIPCHK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
+        s->last_ms[0] = a;
+        s->last_ms[1] = b;
+        s->last_ms[2] = c;
+    }
+    return 0;
+}
+
+extern "C" int ecm_sim_step(ecm_sim* s, int n_steps) {
+    if (!s || n_steps < 0) return fail(ECM_E_ARG, "bad step arguments");
+    if (!s->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before step");
+    (void)hipSetDevice(s->dev);
+    for (int i = 0; i < n_steps; ++i) {
+        int rc = step_once(s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int ecm_sim_sync(ecm_sim* s) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static int component_buf(ecm_sim* s, int c, void** p, size_t* elem) {
+    switch (c) {
+        case ECM_C_POSITION: *p = s->pos[s->cur].p; *elem = 8; return 0;
+        case ECM_C_VELOCITY: *p = s->vel[s->cur].p; *elem = 8; return 0;
+        case ECM_C_PREF_VELOCITY: *p = s->pref_vel.p; *elem = 8; return 0;
+        case ECM_C_ATTRACTION: *p = s->attraction.p; *elem = 8; return 0;
+        case ECM_C_CLEARANCE: *p = s->radius.p; *elem = 4; return 0;
+        case ECM_C_PREF_SPEED: *p = s->pref_speed.p; *elem = 4; return 0;
+        case ECM_C_ACTIVE: *p = s->active.p; *elem = 1; return 0;
+        case ECM_C_ORCA_VELOCITY:
+            if (!s->v_orca.p) return fail(ECM_E_STATE, "keep_orca_velocity was not set");
+            *p = s->v_orca.p; *elem = 8; return 0;
+        default: return fail(ECM_E_ARG, "unknown component %d", c);
+    }
+}
+
+extern "C" int ecm_sim_read(ecm_sim* s, int c, void* dst, size_t bytes) {
+    if (!s || !dst) return fail(ECM_E_ARG, "null argument");
+    void* p;
+    size_t el;
+    int rc = component_buf(s, c, &p, &el);
+    if (rc) return rc;
+    if (bytes != el * (size_t)s->prm.max_agents) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * (size_t)s->prm.max_agents);
+    (void)hipSetDevice(s->dev);
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
+    if (!s || !src) return fail(ECM_E_ARG, "null argument");
+    void* p;
+    size_t el;
+    int rc = component_buf(s, c, &p, &el);
+    if (rc) return rc;
+    const size_t N = (size_t)s->prm.max_agents;
+    if (bytes != el * N) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * N);
+    (void)hipSetDevice(s->dev);
+    HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const float* f = (const float*)src;
+    if (c == ECM_C_CLEARANCE) for (size_t i = 0; i < N; ++i) s->h_radius[i] = f[i];
+    if (c == ECM_C_PREF_SPEED) for (size_t i = 0; i < N; ++i) s->h_speed[i] = f[i];
+    if (c == ECM_C_ACTIVE) {
+        // rebuild the pool: inactive slots, lowest on top
+        const uint8_t* a = (const uint8_t*)src;
+        s->free_stack.clear();
+        s->num_agents = 0;
+        s->last_idx = -1;
+        for (int i = (int)N - 1; i >= 0; --i) {
+            s->h_active[i] = a[i] ? 1 : 0;
+            if (a[i]) { ++s->num_agents; s->last_idx = std::max(s->last_idx, i); }
+            else s->free_stack.push_back(i);
+        }
+    }
+    if (c == ECM_C_POSITION || c == ECM_C_VELOCITY) {
+        // keep both buffers equal so inactive slots read back consistently
+        void* other = (c == ECM_C_POSITION) ? (void*)s->pos[s->cur ^ 1].p : (void*)s->vel[s->cur ^ 1].p;
+        HIPCHK(hipMemcpyAsync(other, src, bytes, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return 0;
+}
+
+extern "C" int ecm_sim_device_ptr(ecm_sim* s, int c, void** out) {
+    if (!s || !out) return fail(ECM_E_ARG, "null argument");
+    size_t el;
+    return component_buf(s, c, out, &el);
+}
+
+extern "C" int ecm_sim_num_agents(ecm_sim* s, int* out) {
+    if (!s || !out) return fail(ECM_E_ARG, "null argument");
+    *out = s->num_agents;
+    return 0;
+}
+extern "C" int ecm_sim_last_index(ecm_sim* s, int* out) {
+    if (!s || !out) return fail(ECM_E_ARG, "null argument");
+    *out = s->last_idx;
+    return 0;
+}
+
+extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
+    if (!s || !nbr || !cnt) return fail(ECM_E_ARG, "null argument");
+    if (s->grid_for_n < 0) return fail(ECM_E_STATE, "no step has run");
+    (void)hipSetDevice(s->dev);
+    const size_t N = (size_t)s->prm.max_agents;
+    DBuf<int> dn, dc;
+    int rc = dn.alloc(N * s->prm.k);
+    if (!rc) rc = dc.alloc(N);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(dn.p, 0xff, N * s->prm.k * sizeof(int), s->stream));
+    HIPCHK(hipMemsetAsync(dc.p, 0, N * sizeof(int), s->stream));
+    // the last step read buffer cur^1; sorted arrays are still those of that step
+    DevState d = make_state(s);
+    HIPCHK(launch_export_nbrs(d, dn.p, dc.p, s->stream));
+    HIPCHK(hipMemcpyAsync(nbr, dn.p, N * s->prm.k * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(cnt, dc.p, N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    dn.release();
+    dc.release();
+    return 0;
+}
+
+extern "C" int ecm_sim_read_paths(ecm_sim* s, int* off, int* cnt, float* pool, size_t pool_cap, size_t* pool_used) {
+    if (!s || !pool_used) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    *pool_used = s->path_used;
+    const size_t N = (size_t)s->prm.max_agents;
+    if (off) HIPCHK(hipMemcpy(off, s->path_off.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    if (cnt) HIPCHK(hipMemcpy(cnt, s->path_cnt.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    if (pool) {
+        if (pool_cap < s->path_used) return fail(ECM_E_ARG, "pool buffer too small (%zu points needed)", s->path_used);
+        if (s->path_used) HIPCHK(hipMemcpy(pool, s->path_pool.p, s->path_used * sizeof(float2), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+extern "C" int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n) {
+    if (!s || !n || (cap > 0 && !ids)) return fail(ECM_E_ARG, "null argument");
+    *n = (int)s->h_replans.size();
+    for (int i = 0; i < cap && i < *n; ++i) ids[i] = s->h_replans[i];
+    return 0;
+}
+
+extern "C" int ecm_sim_stats(ecm_sim* s, uint64_t* out6) {
+    if (!s || !out6) return fail(ECM_E_ARG, "null argument");
+    for (int i = 0; i < 6; ++i) out6[i] = s->stat[i];
+    return 0;
+}
+
+extern "C" int ecm_sim_set_timing(ecm_sim* s, int on) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    s->timing = on != 0;
+    return 0;
+}
+extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out3) {
+    if (!s || !out3) return fail(ECM_E_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) out3[i] = s->last_ms[i];
+    return 0;
+}
+
+// ------------------------------------------------------------- snapshot
+static const uint32_t kSnapMagic = 0x45434d53u;   // "ECMS"
+
+template <typename T>
+static bool wr(FILE* f, const T* p, size_t n) { return fwrite(p, sizeof(T), n, f) == n; }
+template <typename T>
+static bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
+
+extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
+    if (!s || !path) return fail(ECM_E_ARG, "null argument");
+    const size_t N = (size_t)s->prm.max_agents;
+    std::vector<float2> pos(N), vel(N), attr(N), pv(N);
+    std::vector<float> rad(N), spd(N);
+    std::vector<uint8_t> act(N);
+    std::vector<int> off(N), cnt(N);
+    std::vector<float2> pool(s->path_used);
+    int rc = ecm_sim_read(s, ECM_C_POSITION, pos.data(), N * 8);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_VELOCITY, vel.data(), N * 8);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_ATTRACTION, attr.data(), N * 8);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_PREF_VELOCITY, pv.data(), N * 8);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_CLEARANCE, rad.data(), N * 4);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_PREF_SPEED, spd.data(), N * 4);
+    if (!rc) rc = ecm_sim_read(s, ECM_C_ACTIVE, act.data(), N);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(off.data(), s->path_off.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), s->path_cnt.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    if (s->path_used) HIPCHK(hipMemcpy(pool.data(), s->path_pool.p, s->path_used * sizeof(float2), hipMemcpyDeviceToHost));
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
+    uint32_t hdr[4] = {kSnapMagic, 1u, (uint32_t)N, (uint32_t)sizeof(ecm_sim_params)};
+    float box[4] = {s->wx0, s->wy0, s->wx1, s->wy1};
+    uint64_t pu = s->path_used, fs = s->free_stack.size();
+    int32_t ints[4] = {s->num_agents, s->last_idx, (int32_t)s->obs_ring_off.size(), (int32_t)(s->ecm_vert.size() / 2)};
+    int32_t ne = (int32_t)(s->ecm_target.size() / 2);
+    bool ok = wr(f, hdr, 4) && wr(f, &s->prm, 1) && wr(f, box, 4) && wr(f, ints, 4) && wr(f, &ne, 1) && wr(f, &pu, 1) &&
+              wr(f, &fs, 1) && wr(f, pos.data(), N) && wr(f, vel.data(), N) && wr(f, attr.data(), N) &&
+              wr(f, pv.data(), N) && wr(f, rad.data(), N) && wr(f, spd.data(), N) && wr(f, act.data(), N) &&
+              wr(f, off.data(), N) && wr(f, cnt.data(), N) && wr(f, pool.data(), pool.size()) &&
+              wr(f, s->free_stack.data(), fs) && wr(f, s->obs_ring_off.data(), s->obs_ring_off.size()) &&
+              wr(f, s->obs_xy.data(), s->obs_xy.size()) && wr(f, s->ecm_vert.data(), s->ecm_vert.size()) &&
+              wr(f, s->ecm_target.data(), s->ecm_target.size()) && wr(f, s->ecm_pts.data(), s->ecm_pts.size());
+    fclose(f);
+    if (!ok) return fail(ECM_E_ARG, "write to %s failed", path);
+    return 0;
+}
+
+extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
+    if (!s || !path) return fail(ECM_E_ARG, "null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
+    uint32_t hdr[4];
+    ecm_sim_params prm;
+    float box[4];
+    int32_t ints[4], ne;
+    uint64_t pu, fs;
+    bool ok = rd(f, hdr, 4) && hdr[0] == kSnapMagic && hdr[1] == 1u && hdr[3] == sizeof(ecm_sim_params) &&
+              rd(f, &prm, 1) && rd(f, box, 4) && rd(f, ints, 4) && rd(f, &ne, 1) && rd(f, &pu, 1) && rd(f, &fs, 1);
+    if (!ok || (int)hdr[2] != s->prm.max_agents || prm.k != s->prm.k) {
+        fclose(f);
+        return fail(ECM_E_ARG, "%s: not a snapshot for this handle (max_agents / k differ)", path);
+    }
+    const size_t N = hdr[2];
+    std::vector<float2> pos(N), vel(N), attr(N), pv(N), pool(pu);
+    std::vector<float> rad(N), spd(N);
+    std::vector<uint8_t> act(N);
+    std::vector<int> off(N), cnt(N), stack(fs), ro(ints[2]);
+    std::vector<float> oxy, vxy, pts;
+    std::vector<int> tgt;
+    ok = rd(f, pos.data(), N) && rd(f, vel.data(), N) && rd(f, attr.data(), N) && rd(f, pv.data(), N) &&
+         rd(f, rad.data(), N) && rd(f, spd.data(), N) && rd(f, act.data(), N) && rd(f, off.data(), N) &&
+         rd(f, cnt.data(), N) && rd(f, pool.data(), pu) && rd(f, stack.data(), fs) && rd(f, ro.data(), ro.size());
+    if (ok) {
+        int nv = ro.empty() ? 0 : ro.back();
+        oxy.resize(2 * (size_t)nv);
+        vxy.resize(2 * (size_t)ints[3]);
+        tgt.resize(2 * (size_t)ne);
+        pts.resize(8 * (size_t)ne);
+        ok = rd(f, oxy.data(), oxy.size()) && rd(f, vxy.data(), vxy.size()) && rd(f, tgt.data(), tgt.size()) &&
+             rd(f, pts.data(), pts.size());
+    }
+    fclose(f);
+    if (!ok) return fail(ECM_E_ARG, "%s: truncated snapshot", path);
+    s->prm = prm;
+    s->wx0 = box[0]; s->wy0 = box[1]; s->wx1 = box[2]; s->wy1 = box[3];
+    int rc = ecm_sim_set_obstacles(s, oxy.data(), ro.data(), (int)ro.size() - 1);
+    if (!rc) rc = ecm_sim_set_ecm(s, ints[3], vxy.data(), ne, tgt.data(), pts.data());
+    if (!rc) rc = ensure_path_capacity(s, pu > s->path_used ? pu - s->path_used : 0);
+    if (rc) return rc;
+    (void)hipSetDevice(s->dev);
+    if (pu) HIPCHK(hipMemcpy(s->path_pool.p, pool.data(), pu * sizeof(float2), hipMemcpyHostToDevice));
+    s->path_used = pu;
+    HIPCHK(hipMemcpy(s->path_off.p, off.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->path_cnt.p, cnt.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    rc = ecm_sim_write(s, ECM_C_POSITION, pos.data(), N * 8);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_VELOCITY, vel.data(), N * 8);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_ATTRACTION, attr.data(), N * 8);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_PREF_VELOCITY, pv.data(), N * 8);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_CLEARANCE, rad.data(), N * 4);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_PREF_SPEED, spd.data(), N * 4);
+    if (!rc) rc = ecm_sim_write(s, ECM_C_ACTIVE, act.data(), N);
+    if (rc) return rc;
+    s->free_stack = stack;
+    s->num_agents = ints[0];
+    s->last_idx = ints[1];
+    for (size_t i = 0; i < N; ++i) {
+        s->h_path_cnt[i] = cnt[i];
+        s->h_goal[i] = cnt[i] > 0 ? pool[off[i] + cnt[i] - 1] : make_float2(0.0f, 0.0f);
+    }
+    s->grid_for_n = -1;
+    return 0;
+}

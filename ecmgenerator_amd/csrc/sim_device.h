@@ -1,0 +1,90 @@
+// sim_device.h -- device-side data layout of the crowd step (shared by the
+// kernels and the host runtime that owns the buffers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ecm {
+
+// Uniform grid over the world box.  Cell (cx, cy) of a point is
+// floor((p - origin) / cell), clamped to [0, side); NaN maps to 0.
+struct GridDesc {
+    float x0, y0;       // origin
+    float cell;         // cell edge (m)
+    float inv_cell;     // 1 / cell
+    int side;           // cells per axis
+    int bits;           // Morton bits per axis (side <= 1 << bits)
+};
+
+// Everything the step kernels read or write.  Agent arrays are indexed by
+// slot id (entity == slot, Simulator.cpp:40); "sorted" arrays by the
+// position of the agent in the Morton-ordered grid.
+struct DevState {
+    // ---- agent components (Simulator.h:179-187) ----
+    int n_slots;
+    const float2* pos_in;   float2* pos_out;     // double-buffered positions
+    const float2* vel_in;   float2* vel_out;     // double-buffered velocities
+    const float* radius;                         // ClearanceComponent
+    const float* pref_speed;                     // SpeedComponent
+    float2* attraction;                          // attraction point (persistent)
+    float2* pref_vel;                            // preferred velocity
+    uint8_t* active;                             // active flags
+    const int* path_off;                         // PathComponent as CSR
+    const int* path_cnt;
+    const float2* path_pts;
+    float2* v_orca;                              // ORCA output velocity (diagnostic, may be null)
+
+    // ---- agent grid (rebuilt every step) ----
+    GridDesc ag;
+    uint32_t* key;          // [n_slots] Morton key or 0xffffffff
+    uint32_t* cell_count;   // [4^bits]   zero between steps
+    uint32_t* cell_start;   // [4^bits + 1] exclusive scan; [4^bits] = active count
+    float4* sorted_pr;      // [n_slots] (x, y, radius, id bits)
+    float2* sorted_v;       // [n_slots] (vx, vy)
+    int* nbr;               // [k][n_slots] neighbour sorted positions (SoA)
+    int* nbr_cnt;           // [n_slots]   by sorted position
+
+    // ---- events ----
+    int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests
+    int* replan_list;     uint32_t* replan_n;      // UpdatePath requests
+    uint32_t* stats;      // [0] 2D-LP failures (3D used), [1] agents stepped,
+                          // [2] locate fallbacks to the exact-y list, [3] obstacle overflow
+
+    // ---- obstacles (Environment::GetObstacles, edge per vertex in scan order) ----
+    int n_obst_edges;
+    const float4* oe0;      // (L.x, L.y, R.x, R.y),  R = next(L)
+    const float4* oe1;      // (prev(L).x, prev(L).y, next(R).x, next(R).y)
+    const uint8_t* oflags;  // bit0 convex(L), bit1 convex(R)
+    GridDesc og;
+    const int* obucket_start;   // [side^2 + 1]
+    const int* obucket_edges;   // ascending edge ids
+
+    // ---- ECM cells (two per edge, ECMCellCollection.cpp:18-46) ----
+    int n_cells;
+    const float4* cellA;    // (q0, q1) = (v(he1.target), boundary.p0)
+    const float4* cellB;    // (q2, q3) = (boundary.p1, v(he0.target))
+    GridDesc eg;
+    const int* ebucket_start;   // [side^2 + 1]
+    const int* ebucket_cells;   // ascending cell ids
+    // exact-y fallback: cells having a vertex whose y equals a given float
+    int yhash_mask;             // table size - 1 (power of two), -1 if empty
+    const uint32_t* yhash_key;  // y bit pattern (+0.0 for zero), 0xffffffff empty
+    const int* yhash_off;       // [table] start into ycells
+    const int* yhash_cnt;       // [table]
+    const int* ycells;          // ascending cell ids per key
+
+    // ---- parameters (ORCA.h:85-103, Simulator.cpp:538-686) ----
+    int k;
+    float time_horizon;
+    float time_horizon_obst;
+    float inv_time_horizon_obst;  // 1.0f / time_horizon_obst (ORCA.cpp:241)
+    float max_speed;
+    int max_speed_from_pref;
+    float dt;
+    float mass_recip;             // 1.0f / mass (Simulator.cpp:623)
+    float arrival_r2;             // 20 * 20
+    float delete_r2;              // 2 * 2
+};
+
+}  // namespace ecm

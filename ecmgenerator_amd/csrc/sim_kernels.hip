@@ -1,0 +1,795 @@
+// sim_kernels.hip -- gfx950 kernels for one Simulator::Update step
+// (Simulator.cpp:314-323).
+//
+// Step pipeline (all on the handle's stream, no host round trip):
+//   1. k_grid_count / scan / k_grid_scatter: counting sort of the active
+//      agents by Morton cell (replaces KDTree::Construct, KDTree.cpp:22-83).
+//   2. k_attr_knn<K>: goal checks + IRM attraction point on the ECM
+//      (Simulator.cpp:538-590, IRMPathFollower.cpp:14-115, ECM.cpp:20-96,
+//      ECMCellCollection.cpp:57-90), steering (Simulator.cpp:638-657) and the
+//      exact k-nearest query (replaces KDTree::KNearestAgents, KDTree.cpp:85-202).
+//   3. k_orca<K, MAXO>: obstacle query (Simulator.cpp:259-292), ORCA
+//      half-planes (ORCA.cpp:60-424) staged in LDS, 2D LP (ORCA.cpp:428-587),
+//      3D LP (ORCA.cpp:592-669), velocity and position integration
+//      (Simulator.cpp:592-635, 676-677) fused.
+// One agent per lane; lanes are ordered along the Morton curve so a
+// wavefront covers a compact patch of the world.
+#include "ecm_geom.h"
+#include "sim_device.h"
+
+#include <limits.h>
+
+namespace ecm {
+
+static constexpr uint32_t kInvalidKey = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t part1by1(uint32_t x) {
+    x &= 0x0000ffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+__device__ __forceinline__ uint32_t morton2(uint32_t cx, uint32_t cy) {
+    return part1by1(cx) | (part1by1(cy) << 1);
+}
+__device__ __forceinline__ uint32_t n_grid_keys(const GridDesc& g) { return 1u << (2 * g.bits); }
+
+// ------------------------------------------------------------ grid build
+__global__ void k_grid_count(DevState s) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n_slots) return;
+    if (!s.active[i]) { s.key[i] = kInvalidKey; return; }
+    float2 p = s.pos_in[i];
+    int cx = grid_cell(p.x, s.ag.x0, s.ag.inv_cell, s.ag.side);
+    int cy = grid_cell(p.y, s.ag.y0, s.ag.inv_cell, s.ag.side);
+    uint32_t m = morton2((uint32_t)cx, (uint32_t)cy);
+    s.key[i] = m;
+    atomicAdd(&s.cell_count[m], 1u);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+// exclusive scan over a 256-thread block
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t wsum[4];
+    uint32_t inc = wave_incl_scan(v);
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int i = 0; i < w; ++i) off += wsum[i];
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + inc - v;
+}
+// 1024 counts per block -> local exclusive offsets + block sums
+__global__ __launch_bounds__(256) void k_scan_local(const uint32_t* cnt, uint32_t* start, uint32_t* bsum, uint32_t n) {
+    uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = (base + q < n) ? cnt[base + q] : 0u;
+    uint32_t loc = a[0] + a[1] + a[2] + a[3];
+    uint32_t tot;
+    uint32_t ex = block_excl_scan256(loc, tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < n) start[base + q] = ex;
+        ex += a[q];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+// single block: exclusive scan of the block sums, grand total to start[n]
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* start, uint32_t n) {
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < nb; c += 1024u) {
+        uint32_t base = c + threadIdx.x * 4u;
+        uint32_t a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = (base + q < nb) ? bsum[base + q] : 0u;
+        uint32_t loc = a[0] + a[1] + a[2] + a[3];
+        uint32_t tot;
+        uint32_t ex = block_excl_scan256(loc, tot) + carry;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (base + q < nb) bsum[base + q] = ex;
+            ex += a[q];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) start[n] = carry;
+}
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* start, const uint32_t* bsum, uint32_t n) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) start[i] += bsum[i >> 10];
+}
+
+// Counting-sort scatter.  Decrementing the counts leaves cell_count zeroed
+// for the next step.
+__global__ void k_grid_scatter(DevState s) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n_slots) return;
+    uint32_t m = s.key[i];
+    if (m == kInvalidKey) return;
+    uint32_t c = atomicSub(&s.cell_count[m], 1u);
+    uint32_t slot = s.cell_start[m] + c - 1u;
+    float2 p = s.pos_in[i];
+    s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
+    s.sorted_v[slot] = s.vel_in[i];
+}
+
+// ------------------------------------------------------------- ECM query
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// First cell, in collection order, whose ring contains p.  Equivalent to the
+// linear scan of ECMCellCollection::PointLocationQueryLinear: a cell can only
+// contain p if p lies in its (margin-expanded) box, except when p.y equals
+// the y of one of its vertices exactly; those cells are listed per y value.
+__device__ int locate_cell(const DevState& s, V2 p, uint32_t& fallback) {
+    if (s.n_cells == 0) return -1;
+    int bx = grid_cell(p.x, s.eg.x0, s.eg.inv_cell, s.eg.side);
+    int by = grid_cell(p.y, s.eg.y0, s.eg.inv_cell, s.eg.side);
+    int b = by * s.eg.side + bx;
+    int i = s.ebucket_start[b], ie = s.ebucket_start[b + 1];
+    int j = 0, je = 0;
+    if (s.yhash_mask >= 0) {
+        uint32_t key = (p.y == 0.0f) ? 0u : __float_as_uint(p.y);
+        uint32_t h = hash_u32(key) & (uint32_t)s.yhash_mask;
+        while (true) {
+            uint32_t kk = s.yhash_key[h];
+            if (kk == 0xffffffffu) break;
+            if (kk == key) { j = s.yhash_off[h]; je = j + s.yhash_cnt[h]; fallback = 1; break; }
+            h = (h + 1u) & (uint32_t)s.yhash_mask;
+        }
+    }
+    while (i < ie || j < je) {
+        int ci = i < ie ? s.ebucket_cells[i] : INT_MAX;
+        int cj = j < je ? s.ycells[j] : INT_MAX;
+        int c;
+        if (ci < cj) { c = ci; ++i; }
+        else if (cj < ci) { c = cj; ++j; }
+        else { c = ci; ++i; ++j; }
+        float4 A = s.cellA[c], B = s.cellB[c];
+        if (quad_contains(p, mk(A.x, A.y), mk(A.z, A.w), mk(B.x, B.y), mk(B.z, B.w))) return c;
+    }
+    return -1;
+}
+
+// ECM::RetractPoint (ECM.cpp:20-96) given the located cell.  Cell 2e holds
+// (p1, he0.cl, he1.cr, p2), cell 2e+1 holds (p1, he0.cr, he1.cl, p2).
+__device__ bool retract(const DevState& s, V2 loc, int cell, V2& out, V2& oa, V2& ob) {
+    int e2 = cell & ~1;
+    float4 A0 = s.cellA[e2], B0 = s.cellB[e2];
+    float4 A1 = s.cellA[e2 + 1], B1 = s.cellB[e2 + 1];
+    V2 p1 = mk(A0.x, A0.y), p2 = mk(B0.z, B0.w);
+    oa = mk(A0.z, A0.w);            // he0.closest_left
+    ob = mk(B0.x, B0.y);            // he1.closest_right
+    V2 o1, o2, ray;
+    if (left_of_segment(p1, p2, loc)) {
+        o1 = oa; o2 = ob;
+        if (approx(o1, o2)) ray = (p1 - o1) + (p2 - o1);
+        else { V2 v = o2 - o1; ray = mk(v.y, -v.x); }
+    } else {
+        o1 = mk(A1.z, A1.w);        // he0.closest_right
+        o2 = mk(B1.x, B1.y);        // he1.closest_left
+        if (approx(o1, o2)) ray = (p1 - o1) + (p2 - o1);
+        else { V2 v = o2 - o1; ray = mk(-v.y, v.x); }
+    }
+    ray = normalize(ray);
+    return ray_segment(loc, ray, p1, p2, out);
+}
+
+// IRMPathFollower::FindAttractionPoint (IRMPathFollower.cpp:14-115)
+__device__ bool irm_attraction(const DevState& s, V2 pos, const float2* path, int np, V2& out, uint32_t& fb) {
+    int cell = locate_cell(s, pos, fb);
+    if (cell < 0) return false;
+    V2 rl = mk(0.0f, 0.0f), oa, ob;
+    if (!retract(s, pos, cell, rl, oa, ob)) return false;
+    V2 co = closest_on_segment(rl, oa, ob);
+    float clearance = len(rl - co);
+    float clr2 = clearance * clearance;
+    float2 g = path[np - 1];
+    V2 goal = mk(g.x, g.y);
+    if (len2(goal - rl) < clr2) { out = goal; return true; }
+    bool success = false;
+    float2 a = path[0];
+    for (int i = 0; i < np - 1; ++i) {
+        float2 b = path[i + 1];
+        V2 q1 = mk(a.x, a.y) - rl;
+        V2 q2 = mk(b.x, b.y) - rl;
+        a = b;
+        V2 ed = q2 - q1;
+        float el2 = len2(ed);
+        float dm = det(q1, q2);
+        float disc = clr2 * el2 - dm * dm;
+        if (disc < kEps) continue;
+        success = true;
+        float sgn = ed.y < 0.0f ? -1.0f : 1.0f;
+        float sq = sqrtf(disc);
+        V2 i1 = mk((dm * ed.y + sgn * ed.x * sq) / el2, (-dm * ed.x + fabsf(ed.y) * sq) / el2);
+        V2 i2 = mk((dm * ed.y - sgn * ed.x * sq) / el2, (-dm * ed.x - fabsf(ed.y) * sq) / el2);
+        V2 g1 = q1 + rl, g2 = q2 + rl;
+        V2 gi1 = i1 + rl, gi2 = i2 + rl;
+        V2 edg = g2 - g1;
+        float t1 = dot(gi1 - g1, edg) / el2;
+        float t2 = dot(gi2 - g1, edg) / el2;
+        float maxT = -1.0f;
+        if (t1 >= 0.0f && t1 <= 1.0f) { out = gi1; maxT = t1; }
+        if (t2 >= 0.0f && t2 <= 1.0f && t2 > maxT) out = gi2;
+    }
+    return success;
+}
+
+// ---------------------------------------------------------- exact k-NN
+// (d2, id) lexicographic order; d2 as KDTree.cpp:106-108.
+__device__ __forceinline__ bool nb_better(float d, int id, float d2, int id2) {
+    return d < d2 || (d == d2 && id < id2);
+}
+
+template <int K>
+struct NbList {
+    float d[K];
+    int id[K];
+    int slot[K];
+    int n;
+    __device__ void init() {
+#pragma unroll
+        for (int i = 0; i < K; ++i) { d[i] = INFINITY; id[i] = INT_MAX; slot[i] = -1; }
+        n = 0;
+    }
+    __device__ void insert(float dd, int ii, int ss) {
+        if (!nb_better(dd, ii, d[K - 1], id[K - 1])) return;
+#pragma unroll
+        for (int i = K - 1; i > 0; --i) {
+            bool before_prev = nb_better(dd, ii, d[i - 1], id[i - 1]);
+            bool before_cur = nb_better(dd, ii, d[i], id[i]);
+            if (before_prev) { d[i] = d[i - 1]; id[i] = id[i - 1]; slot[i] = slot[i - 1]; }
+            else if (before_cur) { d[i] = dd; id[i] = ii; slot[i] = ss; }
+        }
+        if (nb_better(dd, ii, d[0], id[0])) { d[0] = dd; id[0] = ii; slot[0] = ss; }
+        if (n < K) ++n;
+    }
+    __device__ float kth(int k) const {
+        float r = d[0];
+#pragma unroll
+        for (int i = 1; i < K; ++i) r = (i == k - 1) ? d[i] : r;
+        return r;
+    }
+};
+
+template <int K>
+__device__ __forceinline__ void scan_cell(const DevState& s, uint32_t m, V2 p, NbList<K>& L) {
+    uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
+    for (uint32_t j = a; j < b; ++j) {
+        float4 q = s.sorted_pr[j];
+        float dx = q.x - p.x, dy = q.y - p.y;
+        float d2 = dx * dx + dy * dy;
+        if (d2 > kEps) L.insert(d2, __float_as_int(q.w), (int)j);
+    }
+}
+
+// Ring search over the agent grid; stops when the k-th candidate is closer
+// than every unvisited cell (with a 2 mm guard for rounding).
+template <int K>
+__device__ void knn_query(const DevState& s, V2 p, NbList<K>& L) {
+    const GridDesc& g = s.ag;
+    int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
+    int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
+    L.init();
+    for (int r = 0;; ++r) {
+        int xl = cx - r, xh = cx + r, yl = cy - r, yh = cy + r;
+        if (r == 0) {
+            scan_cell<K>(s, morton2(cx, cy), p, L);
+        } else {
+            int xa = max(xl, 0), xb = min(xh, g.side - 1);
+            for (int x = xa; x <= xb; ++x) {
+                if (yl >= 0) scan_cell<K>(s, morton2(x, yl), p, L);
+                if (yh < g.side) scan_cell<K>(s, morton2(x, yh), p, L);
+            }
+            int ya = max(yl + 1, 0), yb = min(yh - 1, g.side - 1);
+            for (int y = ya; y <= yb; ++y) {
+                if (xl >= 0) scan_cell<K>(s, morton2(xl, y), p, L);
+                if (xh < g.side) scan_cell<K>(s, morton2(xh, y), p, L);
+            }
+        }
+        if (xl <= 0 && yl <= 0 && xh >= g.side - 1 && yh >= g.side - 1) break;
+        if (L.n >= s.k) {
+            float b = INFINITY;
+            if (xl > 0) b = fminf(b, p.x - (g.x0 + (float)xl * g.cell));
+            if (yl > 0) b = fminf(b, p.y - (g.y0 + (float)yl * g.cell));
+            if (xh < g.side - 1) b = fminf(b, (g.x0 + (float)(xh + 1) * g.cell) - p.x);
+            if (yh < g.side - 1) b = fminf(b, (g.y0 + (float)(yh + 1) * g.cell) - p.y);
+            b -= 2e-3f;
+            if (b > 0.0f && b * b > L.kth(s.k)) break;
+        }
+    }
+}
+
+// --------------------------------------- attraction + steering + neighbours
+template <int K>
+__global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
+    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)n_active) return;
+    float4 pr = s.sorted_pr[t];
+    const int id = __float_as_int(pr.w);
+    const V2 pos = mk(pr.x, pr.y);
+
+    // UpdateAttractionPointSystem (Simulator.cpp:538-590)
+    const int off = s.path_off[id], np = s.path_cnt[id];
+    const float2* path = s.path_pts + off;
+    float2 gg = path[np - 1];
+    float gx = pos.x - gg.x, gy = pos.y - gg.y;
+    float dGoal = gx * gx + gy * gy;
+    float2 a0 = s.attraction[id];
+    V2 attr = mk(a0.x, a0.y);
+    bool alive = true;
+    uint32_t fb = 0;
+    if (dGoal < s.arrival_r2) {
+        attr = mk(gg.x, gg.y);
+        if (dGoal < s.delete_r2) alive = false;
+    } else {
+        V2 ap = mk(0.0f, 0.0f);
+        if (irm_attraction(s, pos, path, np, ap, fb)) attr = ap;
+        else {
+            uint32_t q = atomicAdd(s.replan_n, 1u);
+            s.replan_list[q] = id;
+        }
+    }
+    s.attraction[id] = make_float2(attr.x, attr.y);
+    if (fb) atomicAdd(&s.stats[2], 1u);
+    if (!alive) {
+        s.active[id] = 0;                                   // DestroyAgent
+        uint32_t q = atomicAdd(s.destroyed_n, 1u);
+        s.destroyed_list[q] = id;
+        s.nbr_cnt[t] = 0;
+        return;
+    }
+    // ApplySteeringForce (Simulator.cpp:638-657)
+    const float speed = s.pref_speed[id];
+    V2 des = normalize(attr - pos) * speed;
+    s.pref_vel[id] = make_float2(des.x, des.y);
+
+    // k nearest (replaces KDTree::KNearestAgents)
+    NbList<K> L;
+    knn_query<K>(s, pos, L);
+    int cnt = min(L.n, s.k);
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = L.slot[i];
+    s.nbr_cnt[t] = cnt;
+}
+
+// ------------------------------------------------------------------- ORCA
+struct Con { V2 n, p; };
+
+__device__ __forceinline__ Con con_ld(const float4* lds, int slot, int tid) {
+    float4 c = lds[slot * 64 + tid];
+    Con r; r.n = mk(c.x, c.y); r.p = mk(c.z, c.w);
+    return r;
+}
+__device__ __forceinline__ void con_st(float4* lds, int slot, int tid, V2 p, V2 n) {
+    lds[slot * 64 + tid] = make_float4(n.x, n.y, p.x, p.y);
+}
+// Constraint::Contains, methodB (ORCA.h:49-60)
+__device__ __forceinline__ bool con_contains(const Con& c, V2 q) { return det(right(c.n), c.p - q) <= 0.0f; }
+__device__ __forceinline__ float std_min(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float std_max(float a, float b) { return (a < b) ? b : a; }
+
+// Obstacle half-plane for edge e (ORCA.cpp:70-333).  Returns false when the
+// reference emits nothing for it (`continue` without push_back).
+__device__ bool obstacle_constraint(const DevState& s, int e, V2 position, V2 velocity, float rad,
+                                    V2& outP, V2& outN) {
+    float4 a = s.oe0[e], b = s.oe1[e];
+    uint8_t fl = s.oflags[e];
+    V2 pL = mk(a.x, a.y), pR = mk(a.z, a.w);
+    V2 pPrevL = mk(b.x, b.y), pNextR = mk(b.z, b.w);
+    bool cvL = fl & 1, cvR = (fl >> 1) & 1;
+
+    V2 rp1 = pL - position;
+    V2 rp2 = pR - position;
+    V2 seg = pR - pL;
+    const float sp = dot(neg(rp1), seg) / len2(seg);
+    const float distSqLine = len2(neg(rp1) - seg * sp);
+    const float distSq1 = len2(rp1);
+    const float distSq2 = len2(rp2);
+    seg = normalize(seg);
+    const float radiusSq = rad * rad;
+
+    if (sp < 0.0f && distSq1 <= radiusSq) {                         // case 1
+        if (!cvL) return false;
+        outP = mk(0.0f, 0.0f); outN = normalized(neg(rp1));
+        return true;
+    } else if (sp > 1.0f && distSq2 <= radiusSq) {                  // case 2
+        V2 rnd = normalize(pNextR - pR);
+        if (cvR && det(rp2, rnd) >= 0.0f) {
+            outP = mk(0.0f, 0.0f); outN = normalized(neg(rp2));
+            return true;
+        }
+        return false;
+    } else if (sp >= 0.0f && sp < 1.0f && distSqLine <= radiusSq) { // case 3
+        outP = mk(0.0f, 0.0f); outN = right(seg);
+        return true;
+    }
+
+    // The obstacle's (L, R) pair, possibly collapsed to one vertex.  With
+    // R := L, next(R) is the original R; with L := R, prev(L) is the
+    // original L.
+    V2 L = pL, R = pR, prevL = pPrevL, nextR = pNextR;
+    bool cL = cvL, cR = cvR;
+    bool same = false;          // obstNeighborL == obstNeighborR (pointer identity)
+    V2 leftLeg, rightLeg;
+    if (sp < 0.0f && distSqLine <= radiusSq) {
+        if (!cvL) return false;
+        R = pL; nextR = pR; cR = cvL; same = true;
+        const float leg1 = sqrtf(distSq1 - radiusSq);
+        leftLeg = mk(rp1.x * leg1 - rp1.y * rad, rp1.x * rad + rp1.y * leg1) / distSq1;
+        rightLeg = mk(rp1.x * leg1 + rp1.y * rad, -rp1.x * rad + rp1.y * leg1) / distSq1;
+    } else if (sp > 1.0f && distSqLine <= radiusSq) {
+        if (!cvR) return false;
+        L = pR; prevL = pL; cL = cvR; same = true;
+        const float leg2 = sqrtf(distSq2 - radiusSq);
+        leftLeg = mk(rp2.x * leg2 - rp2.y * rad, rp2.x * rad + rp2.y * leg2) / distSq2;
+        rightLeg = mk(rp2.x * leg2 + rp2.y * rad, -rp2.x * rad + rp2.y * leg2) / distSq2;
+    } else {
+        if (cvL) {
+            const float leg1 = sqrtf(distSq1 - radiusSq);
+            leftLeg = mk(rp1.x * leg1 - rp1.y * rad, rp1.x * rad + rp1.y * leg1) / distSq1;
+        } else {
+            leftLeg = neg(seg);
+        }
+        if (cvR) {
+            const float leg2 = sqrtf(distSq2 - radiusSq);
+            rightLeg = mk(rp2.x * leg2 + rp2.y * rad, -rp2.x * rad + rp2.y * leg2) / distSq2;
+        } else {
+            rightLeg = seg;
+        }
+    }
+
+    bool leftForeign = false, rightForeign = false;
+    V2 lnd = normalize(L - prevL);
+    if (cL && det(leftLeg, neg(lnd)) >= 0.0f) { leftLeg = neg(lnd); leftForeign = true; }
+    V2 rnd = normalize(nextR - R);
+    if (cR && det(rightLeg, rnd) <= 0.0f) { rightLeg = rnd; rightForeign = true; }
+
+    const float invT = s.inv_time_horizon_obst;
+    const V2 leftCutoff = (L - position) * invT;
+    const V2 rightCutoff = (R - position) * invT;
+    const V2 cutoffVec = rightCutoff - leftCutoff;
+    const float t = same ? 0.5f : dot(velocity - leftCutoff, cutoffVec) / len2(cutoffVec);
+    const float tLeft = dot(velocity - leftCutoff, leftLeg);
+    const float tRight = dot(velocity - rightCutoff, rightLeg);
+
+    if ((t < 0.0f && tLeft < 0.0f) || (same && tLeft < 0.0f && tRight < 0.0f)) {
+        V2 uw = normalized(velocity - leftCutoff);
+        outP = leftCutoff + (uw * invT) * rad; outN = uw;
+        return true;
+    } else if (t > 1.0f && tRight < 0.0f) {
+        V2 uw = normalized(velocity - rightCutoff);
+        outP = rightCutoff + (uw * invT) * rad; outN = uw;
+        return true;
+    }
+    const float inf = INFINITY;
+    const float dCut = (t < 0.0f || t > 1.0f || same) ? inf : len2(velocity - (leftCutoff + cutoffVec * t));
+    const float dLeft = (tLeft < 0.0f) ? inf : len2(velocity - (leftCutoff + leftLeg * tLeft));
+    const float dRight = (tRight < 0.0f) ? inf : len2(velocity - (rightCutoff + rightLeg * tRight));
+    if (dCut <= dLeft && dCut <= dRight) {
+        V2 nrm = left(neg(seg));
+        outP = leftCutoff + (nrm * invT) * rad; outN = nrm;
+        return true;
+    } else if (dLeft <= dRight) {
+        if (leftForeign) return false;
+        V2 nrm = left(leftLeg);
+        outP = leftCutoff + (nrm * rad) * invT; outN = nrm;
+        return true;
+    } else {
+        if (rightForeign) return false;
+        V2 nrm = right(rightLeg);
+        outP = rightCutoff + (nrm * rad) * invT; outN = nrm;
+        return true;
+    }
+}
+
+// Agent half-plane (ORCA.cpp:339-423)
+__device__ void agent_constraint(const DevState& s, V2 position, V2 velocity, float rad,
+                                 float4 q, float2 qv, V2& outP, V2& outN) {
+    const float T = s.time_horizon;
+    V2 VOPos = mk((q.x - position.x) / T, (q.y - position.y) / T);
+    float VOPosLength = len(VOPos);
+    float combined = q.z + rad;
+    float VORadius = combined / T;
+    V2 relVel = mk(velocity.x - qv.x, velocity.y - qv.y);
+    V2 relPos = mk(q.x - position.x, q.y - position.y);
+    float relPosLength = len(relPos);
+    if (relPosLength < combined) {
+        const V2 w = relVel - relPos / s.dt;
+        const float wl = len(w);
+        const V2 uw = w / wl;
+        const V2 U = uw * (combined / s.dt - wl);
+        outP = velocity + U * 0.5f; outN = uw;
+        return;
+    }
+    float half = atan_f(VORadius / VOPosLength);
+    V2 VOLeftLeg = rotate(VOPos, half);
+    V2 VORightLeg = rotate(VOPos, -half);
+    float sqFromCentre = sq_dist(VOPos, relVel);
+    bool below = det(VOLeftLeg - VOPos, relVel - VOPos) > 0;
+    if (below) {
+        float distToEdge = VORadius - sqrtf(sqFromCentre);
+        V2 ln = normalize(relVel - VOPos);
+        outP = velocity + (ln * distToEdge) * 0.5f; outN = ln;
+    } else {
+        const V2 lperp = left(VOPos) / VOPosLength;
+        if (dot(lperp, relVel) >= 0) {
+            const V2 ll = normalized(VOLeftLeg);
+            const V2 U = ll * dot(relVel, ll) - relVel;
+            outP = velocity + U * 0.5f; outN = left(ll);
+        } else {
+            const V2 rl = normalized(VORightLeg);
+            const V2 U = rl * dot(relVel, rl) - relVel;
+            outP = velocity + U * 0.5f; outN = right(rl);
+        }
+    }
+}
+
+// ORCA::RandomizedLP (ORCA.cpp:428-587) over a virtual list: entries
+// [0, n_direct) at LDS slots [0, n_direct), entries [n_direct, n) at slots
+// [proj_base, ...).  Returns the failing index or n.
+__device__ int lp2d(const float4* lds, int tid, int n, int n_direct, int proj_base, V2 opt, float maxSpeed,
+                    bool useDir, V2& out) {
+    if (useDir) out = opt * maxSpeed;
+    else if (len(opt) > maxSpeed) out = normalized(opt) * maxSpeed;
+    else out = opt;
+    if (n == 0) return n;
+    for (int i = 0; i < n; ++i) {
+        Con h = con_ld(lds, i < n_direct ? i : proj_base + (i - n_direct), tid);
+        if (con_contains(h, out)) continue;
+        V2 dir = right(h.n);
+        float dpd = dot(dir, h.p);
+        float disc = dpd * dpd + maxSpeed * maxSpeed - dot(h.p, h.p);
+        if (disc <= 0.0f) return i;
+        float sq = sqrtf(disc);
+        float lft = -dpd - sq;
+        float rgt = -dpd + sq;
+        for (int j = 0; j < i; ++j) {
+            Con hj = con_ld(lds, j < n_direct ? j : proj_base + (j - n_direct), tid);
+            float den = det(right(h.n), right(hj.n));
+            float num = det(right(hj.n), h.p - hj.p);
+            if (fabsf(den) <= kEps) {
+                if (num < 0.0f) return i;
+                continue;
+            }
+            const float t = num / den;
+            if (den >= 0.0f) rgt = std_min(rgt, t);
+            else lft = std_max(lft, t);
+            if (lft > rgt) return i;
+        }
+        if (useDir) {
+            if (dot(opt, right(h.n)) > 0) out = h.p + right(h.n) * rgt;
+            else out = h.p + right(h.n) * lft;
+        } else {
+            float t = dot(right(h.n), opt - h.p);
+            if (t < lft) out = h.p + right(h.n) * lft;
+            else if (t > rgt) out = h.p + right(h.n) * rgt;
+            else out = h.p + right(h.n) * t;
+        }
+    }
+    return n;
+}
+
+// ORCA::RandomizedLP3D (ORCA.cpp:592-669); projected agent constraints are
+// written to LDS slots [proj_base, proj_base + i - nObst).
+__device__ void lp3d(float4* lds, int tid, int n, int nObst, int proj_base, float maxSpeed, int failed, V2& out) {
+    float maxPen = 0.0f;
+    for (int i = failed; i < n; ++i) {
+        const Con ci = con_ld(lds, i, tid);
+        V2 dir = right(ci.n);
+        if (det(dir, ci.p - out) <= maxPen) continue;
+        int np = 0;
+        for (int j = nObst; j < i; ++j) {
+            const Con cj = con_ld(lds, j, tid);
+            float d = det(right(ci.n), right(cj.n));
+            V2 newP;
+            if (fabsf(d) <= kEps) {
+                if (dot(ci.n, cj.n) > 0) continue;
+                newP = (ci.p + cj.p) * 0.5f;
+            } else {
+                float t = det(right(cj.n), ci.p - cj.p) / d;
+                newP = ci.p + right(ci.n) * t;
+            }
+            con_st(lds, proj_base + np, tid, newP, normalized(cj.n - ci.n));
+            ++np;
+        }
+        const V2 temp = out;
+        if (lp2d(lds, tid, nObst + np, nObst, proj_base, ci.n, maxSpeed, true, out) < nObst + np) out = temp;
+        maxPen = det(dir, ci.p - out);
+    }
+}
+
+template <int K, int MAXO>
+__global__ __launch_bounds__(64) void k_orca(DevState s) {
+    __shared__ float4 lds[(MAXO + 2 * K) * 64];
+    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x * 64 + tid;
+    if (t >= (int)n_active) return;
+    const float4 pr = s.sorted_pr[t];
+    const float2 v0 = s.sorted_v[t];
+    const int id = __float_as_int(pr.w);
+    if (!s.active[id]) {                 // destroyed this step: state unchanged
+        s.pos_out[id] = make_float2(pr.x, pr.y);
+        s.vel_out[id] = v0;
+        return;
+    }
+    const V2 position = mk(pr.x, pr.y);
+    const V2 velocity = mk(v0.x, v0.y);
+    const float rad = pr.z;
+    const float speed = s.pref_speed[id];
+    const float maxSpeed = s.max_speed_from_pref ? speed : s.max_speed;
+    const float2 pv = s.pref_vel[id];
+
+    // obstacle neighbours in scan order, constraints generated in place
+    int nc = 0;
+    if (s.n_obst_edges > 0) {
+        const float range = s.time_horizon_obst * maxSpeed + rad;          // ORCA.cpp:27
+        const float range2 = range * range;
+        int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
+        int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
+        int b = by * s.og.side + bx;
+        int e0 = s.obucket_start[b], e1 = s.obucket_start[b + 1];
+        for (int q = e0; q < e1; ++q) {
+            int e = s.obucket_edges[q];
+            float4 a = s.oe0[e];
+            V2 L = mk(a.x, a.y), R = mk(a.z, a.w);
+            float sld = det(L - position, R - L);
+            float sqd = sld * sld / sq_dist(L, R);
+            if (sqd < range2 && sld < 0.0f) {
+                V2 c = closest_on_segment(position, L, R);
+                if (sq_dist(c, position) < range2) {
+                    V2 cp, cn;
+                    if (obstacle_constraint(s, e, position, velocity, rad, cp, cn)) {
+                        con_st(lds, nc, tid, cp, cn);
+                        ++nc;
+                    }
+                }
+            }
+        }
+    }
+    const int nObst = nc;
+    const int cnt = s.nbr_cnt[t];
+    for (int j = 0; j < cnt; ++j) {
+        int sj = s.nbr[(size_t)j * s.n_slots + t];
+        float4 q = s.sorted_pr[sj];
+        float2 qv = s.sorted_v[sj];
+        V2 cp, cn;
+        agent_constraint(s, position, velocity, rad, q, qv, cp, cn);
+        con_st(lds, nc, tid, cp, cn);
+        ++nc;
+    }
+
+    V2 o;
+    int fail = lp2d(lds, tid, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
+    bool used3d = fail < nc;
+    if (used3d) lp3d(lds, tid, nc, nObst, MAXO + K, maxSpeed, fail, o);
+    uint64_t bal = __ballot(used3d);
+    if ((tid & 63) == 0 && bal) atomicAdd(&s.stats[0], (uint32_t)__popcll(bal));
+
+    // force, UpdateVelocitySystem, UpdatePositionSystem
+    const V2 F = mk(o.x - velocity.x, o.y - velocity.y);
+    const V2 nv = mk(velocity.x + F.x * s.mass_recip * s.dt, velocity.y + F.y * s.mass_recip * s.dt);
+    const V2 npn = mk(position.x + (nv.x * s.dt), position.y + (nv.y * s.dt));
+    s.pos_out[id] = make_float2(npn.x, npn.y);
+    s.vel_out[id] = make_float2(nv.x, nv.y);
+    if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
+}
+
+// neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
+__global__ void k_export_nbrs(DevState s, int* out, int* out_cnt) {
+    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)n_active) return;
+    int id = __float_as_int(s.sorted_pr[t].w);
+    int cnt = s.nbr_cnt[t];
+    for (int j = 0; j < s.k; ++j)
+        out[(size_t)id * s.k + j] = j < cnt ? __float_as_int(s.sorted_pr[s.nbr[(size_t)j * s.n_slots + t]].w) : -1;
+    out_cnt[id] = cnt;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------- launchers
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) {
+    const uint32_t G = 1u << (2 * s.ag.bits);
+    hipLaunchKernelGGL(k_grid_count, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
+    const uint32_t nb = (G + 1023u) / 1024u;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, st, (const uint32_t*)s.cell_count, s.cell_start, bsum, G);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb, s.cell_start, G);
+    hipLaunchKernelGGL(k_scan_add, dim3(cdiv(G, 256)), dim3(256), 0, st, s.cell_start, (const uint32_t*)bsum, G);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
+    return hipGetLastError();
+}
+
+template <int K>
+static void launch_orca_k(const DevState& s, int maxo, hipStream_t st) {
+    dim3 g(cdiv(s.n_slots, 64)), b(64);
+    if (maxo <= 16) hipLaunchKernelGGL((k_orca<K, 16>), g, b, 0, st, s);
+    else if (maxo <= 32) hipLaunchKernelGGL((k_orca<K, 32>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_orca<K, 64>), g, b, 0, st, s);
+}
+
+hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
+    dim3 g(cdiv(s.n_slots, 256)), b(256);
+    if (s.k <= 5) hipLaunchKernelGGL(k_attr_knn<5>, g, b, 0, st, s);
+    else if (s.k <= 10) hipLaunchKernelGGL(k_attr_knn<10>, g, b, 0, st, s);
+    else hipLaunchKernelGGL(k_attr_knn<16>, g, b, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, st);
+    else launch_orca_k<16>(s, maxo, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_export_nbrs, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s, out, out_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace ecm
+
+namespace ecm {
+
+// SpawnAgent's component writes (Simulator.cpp:175-197) for a batch.
+__global__ void k_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
+                                const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
+                                float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
+                                int* path_cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int id = ids[i];
+    if (id < 0) return;
+    pos[id] = start[i];
+    vel[id] = make_float2(0.0f, 0.0f);
+    pref_vel[id] = make_float2(0.0f, 0.0f);
+    radius[id] = rad[i];
+    pref_speed[id] = spd[i];
+    active[id] = 1;
+    path_off[id] = off[i];
+    path_cnt[id] = cnt[i];
+}
+
+hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
+                                const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
+                                float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
+                                int* path_cnt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spawn_scatter, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ids, start, rad, spd, off, cnt,
+                       pos, vel, radius, pref_speed, pref_vel, active, path_off, path_cnt);
+    return hipGetLastError();
+}
+
+}  // namespace ecm

@@ -1,0 +1,53 @@
+"""Shared helpers for the parity tests: build a scene, mirror the GPU state
+for the oracle, and compare step outputs."""
+from __future__ import annotations
+
+import numpy as np
+
+from ecmgenerator_amd import scenarios as S
+from ecmgenerator_amd.simulator import Params, Simulator
+
+
+def small_scene(seed: int, half: float = 50.0, n_rects: int = 8, n_agents: int = 300, side=(2, 10)):
+    rng = np.random.default_rng(seed)
+    box = (-half, -half, half, half)
+    rects = S.random_rects(rng, box, n_rects, side[0], side[1])
+    obs = S.rects_to_obstacles(rects)
+    ecm = S.empty_square_ecm(*box)
+    agents = S.straight_agents(rng, box, n_agents, rects)
+    return box, rects, obs, ecm, agents
+
+
+def oracle_state(sim: Simulator) -> dict:
+    st = sim.state()
+    off, cnt, pool = sim.paths()
+    st = dict(st)
+    st["path_off"] = off
+    st["path_cnt"] = np.maximum(cnt, 1)
+    st["path_xy"] = pool if pool.shape[0] else np.zeros((1, 2), np.float32)
+    return st
+
+
+def compare_step(sim: Simulator, before: dict, ora: dict, check_orca=True):
+    """Bitwise comparison of one GPU step against the oracle step from the
+    same input state.  Returns a dict of mismatch counts."""
+    after = sim.state()
+    act0 = before["active"].astype(bool)
+    res = {}
+    for key in ("position", "velocity", "attraction"):
+        a, b = after[key][act0], ora[key][act0]
+        res[key] = int(np.sum(~((a == b) | (np.isnan(a) & np.isnan(b))).all(axis=1)))
+    alive = act0 & ora["active"].astype(bool)
+    a, b = after["pref_velocity"][alive], ora["pref_velocity"][alive]
+    res["pref_velocity"] = int(np.sum(~((a == b) | (np.isnan(a) & np.isnan(b))).all(axis=1)))
+    res["active"] = int(np.sum(after["active"][act0] != ora["active"][act0]))
+    nbr, cnt = sim.neighbors()
+    res["nbr_cnt"] = int(np.sum(cnt[alive] != ora["nbr_cnt"][alive]))
+    res["nbr"] = int(np.sum((nbr[alive] != ora["nbr"][alive]).any(axis=1)))
+    if check_orca and sim.params.keep_orca_velocity:
+        a, b = after["orca_velocity"][alive], ora["v_orca"][alive]
+        res["v_orca"] = int(np.sum(~((a == b) | (np.isnan(a) & np.isnan(b))).all(axis=1)))
+    rp = np.zeros(sim.max_agents, np.uint8)
+    rp[sim.replans()] = 1
+    res["replan"] = int(np.sum(rp[act0] != ora["replan"][act0]))
+    return res

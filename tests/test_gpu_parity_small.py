@@ -1,0 +1,51 @@
+"""GPU step vs CPU oracle, bit for bit, from identical input states, on
+small seeded scenes (one agent per lane path through the C ABI)."""
+import numpy as np
+import pytest
+
+from ecmgenerator_amd.simulator import Params, Simulator
+from helpers import compare_step, oracle_state, small_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def run_parity(seed, params, n_agents=300, steps=(0, 1, 2, 5, 10, 20), n_rects=8, half=50.0):
+    from oracle import oracle_py as O
+    box, rects, obs, ecm, agents = small_scene(seed, half=half, n_rects=n_rects, n_agents=n_agents)
+    sim = Simulator(ecm, obs, box, params)
+    ids = sim.SpawnAgents(agents)
+    assert (ids >= 0).sum() > 0.9 * agents.n
+    worst = {}
+    last = max(steps)
+    for it in range(last + 1):
+        if it in steps:
+            before = oracle_state(sim)
+            ora = O.step(params, before, obs, ecm)
+            sim.Update()
+            res = compare_step(sim, before, ora)
+            print(f"seed={seed} step={it} {res}", flush=True)
+            for k, v in res.items():
+                worst[k] = max(worst.get(k, 0), v)
+        else:
+            sim.Update()
+    sim.close()
+    return worst
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_parity_default_params(seed):
+    p = Params(max_agents=512, keep_orca_velocity=True)
+    worst = run_parity(seed, p)
+    assert all(v == 0 for v in worst.values()), worst
+
+
+def test_parity_reference_literal_params():
+    p = Params.reference(max_agents=512, keep_orca_velocity=True)
+    worst = run_parity(7, p)
+    assert all(v == 0 for v in worst.values()), worst
+
+
+def test_parity_dense_crowd():
+    p = Params(max_agents=2048, keep_orca_velocity=True)
+    worst = run_parity(11, p, n_agents=1800, half=30.0, n_rects=4, steps=(0, 3, 8, 15))
+    assert all(v == 0 for v in worst.values()), worst
