@@ -85,8 +85,10 @@ SIGNATURES = {
     "ecm_scene_plan_paths": (_I, [_P, _I, _PF, _PF, _PF, _I, _PI, _PF, C.c_longlong, _PI, _I]),
     "ecm_scene_inside_obstacle": (_I, [_P, _F, _F]),
     "ecm_scene_replan_callback": (REPLAN_FN, []),
-    "ecm_scenario_generate": (_I, [_I, C.c_uint64, _I, C.POINTER(_P), _PI]),
-    "ecm_scenario_agents": (_I, [_P, _I, _PF, _PF, _PF, _PF]),
+    "ecm_scenario_generate": (_I, [_I, C.c_uint64, _I, C.POINTER(_P), C.POINTER(_P)]),
+    "ecm_scenario_counts": (_I, [_P, _PI, C.POINTER(C.c_longlong)]),
+    "ecm_scenario_export": (_I, [_P, _PF, _PF, _PF, _PF, _PI, _PI, _PF]),
+    "ecm_scenario_destroy": (None, [_P]),
 }
 
 

@@ -1,9 +1,9 @@
 """Named benchmark / test workloads.
 
-Each workload builds (box, obstacles, ECM, agents, params).  `proxy-*`
-workloads use the closed-form ECM of the empty box and straight two-point
-paths; they exercise the full step at the configured sizes but with a
-trivial ECM.
+Each workload builds (box, obstacles, ECM, agents, params).  c0 / c1 / c2
+come from the seeded scenario generator (ECM built from the obstacles,
+paths from the ECM planner).  `proxy-*` workloads use the closed-form ECM
+of the empty box and straight two-point paths.
 """
 from __future__ import annotations
 
@@ -48,7 +48,31 @@ def _proxy(name, half, n_rects, side, n_agents, seed):
     return Workload(name, box, obs, ecm, agents, params, info={"ecm": "empty-box closed form", "paths": "straight"})
 
 
+def _scenario(name, config, seed, n_agents=0, params=None):
+    from . import scene as SC
+    sc, agents, goals = SC.generate(config, seed=seed, n_agents=n_agents)
+    ecm = sc.ecm()
+    obs = sc.obstacles()
+    p = params or Params()
+    p.max_agents = agents.n
+    return Workload(name, sc.box, obs, ecm, agents, p, scene=sc,
+                    info={"ecm": "built (segment Voronoi of the rectangles)", "paths": "ECM planner",
+                          "seed": seed})
+
+
 def build(name: str, seed: int = 1) -> Workload:
+    """c0: 200 m, 40 obstacles, 5k agents; c1: 1000 m, 400 obstacles, 100k
+    agents; c2: 2000 m, 1,500 obstacles (barrier with 64 gaps of 8 m), two
+    opposing flows of 500k agents; c0-ref: c0 with the reference's literal
+    parameters (k = 5, T = 10 s, maxSpeed = preferred speed)."""
+    if name == "c0":
+        return _scenario(name, 0, seed)
+    if name == "c0-ref":
+        return _scenario(name, 0, seed, params=Params.reference())
+    if name == "c1":
+        return _scenario(name, 1, seed)
+    if name == "c2":
+        return _scenario(name, 2, seed)
     if name == "proxy-c0":
         return _proxy(name, 100.0, 40, (2, 15), 5000, seed)
     if name == "proxy-c1":
@@ -58,4 +82,4 @@ def build(name: str, seed: int = 1) -> Workload:
     raise KeyError(f"unknown workload {name}")
 
 
-DEFAULT_BENCH = "proxy-c2"
+DEFAULT_BENCH = "c2"

@@ -28,11 +28,16 @@ def oracle_state(sim: Simulator) -> dict:
     return st
 
 
-def compare_step(sim: Simulator, before: dict, ora: dict, check_orca=True):
+def compare_step(sim: Simulator, before: dict, ora: dict, check_orca=True, subset=None, detail=False):
     """Bitwise comparison of one GPU step against the oracle step from the
-    same input state.  Returns a dict of mismatch counts."""
+    same input state (restricted to `subset` when given).  Returns a dict of
+    mismatch counts."""
     after = sim.state()
     act0 = before["active"].astype(bool)
+    if subset is not None:
+        m = np.zeros_like(act0)
+        m[subset] = True
+        act0 = act0 & m
     res = {}
     for key in ("position", "velocity", "attraction"):
         a, b = after[key][act0], ora[key][act0]
@@ -50,4 +55,7 @@ def compare_step(sim: Simulator, before: dict, ora: dict, check_orca=True):
     rp = np.zeros(sim.max_agents, np.uint8)
     rp[sim.replans()] = 1
     res["replan"] = int(np.sum(rp[act0] != ora["replan"][act0]))
+    if detail:
+        bad = np.flatnonzero(act0 & ~((after["velocity"] == ora["velocity"]) | (np.isnan(after["velocity"]) & np.isnan(ora["velocity"]))).all(axis=1))
+        res["bad_ids"] = bad[:20].tolist()
     return res
