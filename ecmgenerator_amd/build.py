@@ -65,6 +65,17 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
+    # C++ driver of the Simulator facade (run by a GPU test)
+    drv = os.path.join(HERE, "..", "tests", "cpp", "facade_drive.cpp")
+    if os.path.exists(drv):
+        outdir = os.path.join(HERE, "..", "tests", "cpp", "_build")
+        os.makedirs(outdir, exist_ok=True)
+        out = os.path.join(outdir, "facade_drive")
+        cmd = [HIPCC, "-O2", "-std=c++17", drv, "-o", out, "-L", HERE, "-lecmcrowd",
+               "-Wl,-rpath,$ORIGIN/../../../ecmgenerator_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"facade driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB

@@ -1,0 +1,48 @@
+// Drives the C++17 Simulator facade the way ECMApplication drives the
+// reference (ECMApplication/main.cpp:50-54, Application.cpp:143): build an
+// environment, spawn agents, call Update once per frame, read the SoA.
+// Exit code 0 when every check holds.
+#include <cmath>
+#include <cstdio>
+
+#include "../../ecmgenerator_amd/host/Simulator.hpp"
+
+int main() {
+    ecm_scene* scene = nullptr;
+    ecm_scenario* sc = nullptr;
+    if (ecm_scenario_generate(3, 7, 200, &scene, &sc) != 0) { std::printf("scenario failed: %s\n", ecm_last_error()); return 2; }
+    int n = 0;
+    long long np = 0;
+    ecm_scenario_counts(sc, &n, &np);
+    std::vector<float> st(2 * n), go(2 * n), r(n), s(n), path(2 * np);
+    std::vector<int> off(n), cnt(n);
+    ecm_scenario_export(sc, st.data(), go.data(), r.data(), s.data(), off.data(), cnt.data(), path.data());
+    ecm_amd::Simulation::Simulator sim(scene, -30, -30, 30, 30, 256, 0.1f);
+    sim.Initialize();
+    int spawned = 0;
+    for (int i = 0; i < n; ++i) spawned += sim.SpawnAgent(st[2 * i], st[2 * i + 1], go[2 * i], go[2 * i + 1], r[i], s[i]) >= 0;
+    std::printf("spawned %d of %d, GetNumAgents %d, GetLastIndex %d\n", spawned, n, sim.GetNumAgents(), sim.GetLastIndex());
+    if (spawned < n / 2 || sim.GetNumAgents() != spawned) return 3;
+    std::vector<float> x0(2 * 256);
+    auto* p0 = sim.GetPositionData();
+    for (int i = 0; i < 256; ++i) { x0[2 * i] = p0[i].x; x0[2 * i + 1] = p0[i].y; }
+    for (int f = 0; f < 50; ++f) sim.Update(1.0f / 60.0f);
+    auto* p = sim.GetPositionData();
+    auto* v = sim.GetVelocityData();
+    bool* act = sim.GetActiveFlags();
+    double moved = 0;
+    int bad = 0, alive = 0;
+    for (int i = 0; i < 256; ++i) {
+        if (!std::isfinite(p[i].x) || !std::isfinite(v[i].dx)) ++bad;
+        moved += std::hypot(p[i].x - x0[2 * i], p[i].y - x0[2 * i + 1]);
+        alive += act[i] ? 1 : 0;
+    }
+    std::printf("after 50 steps: alive %d, mean displacement %.3f m, non-finite %d\n", alive, moved / spawned, bad);
+    ecm_scenario_destroy(sc);
+    if (bad || moved / spawned < 1.0) return 4;
+    sim.Reset();
+    if (sim.GetNumAgents() != 0) return 5;
+    std::printf("facade ok\n");
+    ecm_scene_destroy(scene);
+    return 0;
+}

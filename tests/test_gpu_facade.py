@@ -1,0 +1,20 @@
+"""The C++17 Simulator facade (ecmgenerator_amd/host/Simulator.hpp) driven
+like ECMApplication drives the reference: spawn, Update per frame, read the
+component arrays, Reset."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cpp_facade_drive(native_gpu):
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "facade_drive")
+    assert os.path.exists(exe), "build() compiles tests/cpp/facade_drive.cpp"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "facade ok" in r.stdout
