@@ -34,7 +34,24 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-agents", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (reference for HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mb: int) -> None:
+    """One hipMemcpy device-to-device of `mb` MiB: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE give the counter-to-bytes factor."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0          # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def main():
@@ -61,21 +78,26 @@ def main():
     for _ in range(args.warmup):
         sim.Update()
     sim.sync()
-    sim.set_timing(True)
     st0 = sim.stats()
-    kt = np.zeros(3)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.Update()
-        kt += np.array(sim.last_timing())
     sim.sync()
     t1 = time.perf_counter()
     st1 = sim.stats()
+    # per-kernel HIP-event timing on the handle's stream, separate steps
+    sim.set_timing(True)
+    kt = np.zeros(3)
+    n_kt = max(3, min(args.steps, 10))
+    for _ in range(n_kt):
+        sim.Update()
+        kt += np.array(sim.last_timing())
+    sim.set_timing(False)
     updates = st1["agent_updates"] - st0["agent_updates"]
     wall = t1 - t0
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / wall
-    kt_mean = kt / args.steps
+    kt_mean = kt / n_kt
     names = ["grid_build", "attraction_knn", "orca_lp_integrate"]
     dom = int(np.argmax(kt_mean))
     n_per_step = updates / args.steps
@@ -90,6 +112,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": names[dom],
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
+    if args.copy_calibration_mb:
+        copy_calibration(args.copy_calibration_mb)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sim, wl, args.cpu_sample_agents)

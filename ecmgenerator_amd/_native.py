@@ -54,6 +54,7 @@ SIGNATURES = {
     "ecm_sim_reset": (_I, [_P]),
     "ecm_sim_set_path": (_I, [_P, _I, _PF, _I]),
     "ecm_sim_set_replan_callback": (_I, [_P, REPLAN_FN, _P]),
+    "ecm_sim_set_replan_threads": (_I, [_P, _I]),
     "ecm_sim_step": (_I, [_P, _I]),
     "ecm_sim_sync": (_I, [_P]),
     "ecm_sim_read": (_I, [_P, _I, _P, C.c_size_t]),

@@ -12,19 +12,23 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, uint32_t q_base, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
                                 const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
                                 float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
                                 int* path_cnt, hipStream_t st);
+hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt,
+                               hipStream_t st);
 }  // namespace ecm
 
 using namespace ecm;
@@ -118,6 +122,10 @@ struct ecm_sim {
 
     // events (device) + pinned mirror of the counters
     DBuf<int> destroyed_list, replan_list;
+    DBuf<float2> replan_pos;
+    static const int kReplanPeek = 1024;     // replan requests read back while k_orca runs
+    int* h_replan_ids = nullptr;             // pinned [kReplanPeek]
+    float2* h_replan_pos = nullptr;          // pinned [kReplanPeek]
     DBuf<uint32_t> counters;     // [0] destroyed_n [1] replan_n [2..5] stats
     uint32_t* h_counters = nullptr;
 
@@ -131,6 +139,14 @@ struct ecm_sim {
     DBuf<int> obucket_start, obucket_edges;
     int maxo = 16;
     float obst_range_built = -1.0f;
+    float max_radius = 0.0f, max_pref_speed = 0.0f;   // over every agent spawned
+
+    // 3D-LP queue (capacity max_agents)
+    DBuf<int> q3d_agent;
+    DBuf<int4> q3d_info;
+    DBuf<float2> q3d_vel;
+    DBuf<float4> q3d_con;
+    int q3d_stride = 0;
 
     // ECM
     std::vector<float> ecm_vert;
@@ -157,9 +173,11 @@ struct ecm_sim {
     // callbacks, stats, timing
     ecm_replan_fn replan_fn = nullptr;
     void* replan_user = nullptr;
+    int replan_threads = 1;
     uint64_t stat[6] = {0, 0, 0, 0, 0, 0};
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_replan = nullptr;
     float last_ms[3] = {0, 0, 0};
     std::vector<int> h_replans;
 };
@@ -197,13 +215,21 @@ static void free_all(ecm_sim* s) {
     s->key.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
     s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
+    s->q3d_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     s->oe0.release(); s->oe1.release(); s->oflags.release();
     s->obucket_start.release(); s->obucket_edges.release();
     s->cellA.release(); s->cellB.release(); s->ebucket_start.release(); s->ebucket_cells.release();
     s->yhash_key.release(); s->yhash_off.release(); s->yhash_cnt.release(); s->ycells.release();
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     s->h_counters = nullptr;
+    if (s->h_replan_ids) (void)hipHostFree(s->h_replan_ids);
+    if (s->h_replan_pos) (void)hipHostFree(s->h_replan_pos);
+    s->h_replan_ids = nullptr;
+    s->h_replan_pos = nullptr;
+    s->replan_pos.release();
     for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (s->ev_replan) (void)hipEventDestroy(s->ev_replan);
+    s->ev_replan = nullptr;
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -256,10 +282,17 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->destroyed_list.alloc(N);
     if (!rc) rc |= s->replan_list.alloc(N);
     if (!rc) rc |= s->counters.alloc(8);
+    if (!rc) rc |= s->replan_pos.alloc(N);
     if (!rc && hipHostMalloc((void**)&s->h_counters, 8 * sizeof(uint32_t)) != hipSuccess)
+        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
+    if (!rc && hipHostMalloc((void**)&s->h_replan_ids, ecm_sim::kReplanPeek * sizeof(int)) != hipSuccess)
+        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
+    if (!rc && hipHostMalloc((void**)&s->h_replan_pos, ecm_sim::kReplanPeek * sizeof(float2)) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     for (int i = 0; i < 4 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
+    if (!rc && hipEventCreateWithFlags(&s->ev_replan, hipEventDisableTiming) != hipSuccess)
+        rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc) {
         for (int b = 0; b < 2 && !rc; ++b) {
             rc |= zero_dev(s->pos[b].p, N * sizeof(float2), s->stream);
@@ -290,15 +323,11 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
 }
 
 // ------------------------------------------------------------- obstacles
+// Largest obstacle query range of any agent ever spawned (ORCA.cpp:27),
+// a conservative bound for the obstacle buckets.
 static float obstacle_range(const ecm_sim* s) {
-    float rmax = 0.0f, vmax = s->prm.max_speed;
-    for (size_t i = 0; i < s->h_active.size(); ++i) {
-        if (!s->h_active[i]) continue;
-        rmax = std::max(rmax, s->h_radius[i]);
-        if (s->prm.max_speed_from_pref) vmax = std::max(vmax, s->h_speed[i]);
-    }
-    if (s->prm.max_speed_from_pref && s->num_agents == 0) vmax = 0.0f;
-    return s->prm.time_horizon_obst * vmax + rmax;
+    float vmax = s->prm.max_speed_from_pref ? s->max_pref_speed : s->prm.max_speed;
+    return s->prm.time_horizon_obst * vmax + s->max_radius;
 }
 
 // squared distance from segment ab to the box [bx0,bx1]x[by0,by1] (double)
@@ -676,6 +705,8 @@ extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const flo
         s->h_active[id] = 1;
         s->h_radius[id] = radius[i];
         s->h_speed[id] = pref_speed[i];
+        s->max_radius = std::max(s->max_radius, radius[i]);
+        s->max_pref_speed = std::max(s->max_pref_speed, pref_speed[i]);
         int c = path_cnt[i];
         offs[i] = (int)(s->path_used + pts.size());
         for (int q = 0; q < c; ++q) pts.push_back(make_float2(path_xy[2 * ((size_t)path_off[i] + q)], path_xy[2 * ((size_t)path_off[i] + q) + 1]));
@@ -764,6 +795,12 @@ extern "C" int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* u
     return 0;
 }
 
+extern "C" int ecm_sim_set_replan_threads(ecm_sim* s, int n) {
+    if (!s || n < 1) return fail(ECM_E_ARG, "bad replan thread count");
+    s->replan_threads = n;
+    return 0;
+}
+
 // ------------------------------------------------------------------ step
 static int ensure_agent_grid(ecm_sim* s) {
     int n = std::max(1, s->num_agents);
@@ -806,7 +843,10 @@ static DevState make_state(ecm_sim* s) {
     d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
     d.destroyed_list = s->destroyed_list.p; d.destroyed_n = s->counters.p;
     d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
+    d.replan_pos = s->replan_pos.p;
     d.stats = s->counters.p + 2;
+    d.q3d_agent = s->q3d_agent.p; d.q3d_info = s->q3d_info.p; d.q3d_vel = s->q3d_vel.p;
+    d.q3d_con = s->q3d_con.p; d.con_stride = s->q3d_stride;
     d.n_obst_edges = s->n_obst_edges;
     d.oe0 = s->oe0.p; d.oe1 = s->oe1.p; d.oflags = s->oflags.p; d.og = s->og;
     d.obucket_start = s->obucket_start.p; d.obucket_edges = s->obucket_edges.p;
@@ -827,6 +867,64 @@ static DevState make_state(ecm_sim* s) {
     return d;
 }
 
+// UpdatePath (Simulator.cpp:97-124, 579-587) for the agents whose IRM
+// failed: planned from their step-start position to their goal.  Runs on
+// replan_threads threads while the GPU finishes the step; the new paths
+// are uploaded stream-ordered, before the next step reads them.
+static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vector<float2>& rpos) {
+    const int n = (int)rid.size();
+    const int cap = 4096;
+    std::vector<std::vector<float>> res(n);
+    std::vector<int> cnt(n, 0);
+    auto work = [&](int i) {
+        int id = rid[i];
+        res[i].resize(2 * cap);
+        float2 g = s->h_goal[id];
+        int np = 0;
+        int r = s->replan_fn(s->replan_user, id, rpos[i].x, rpos[i].y, g.x, g.y, s->h_radius[id], res[i].data(), cap, &np);
+        cnt[i] = (r == 0 && np >= 1) ? np : 0;     // an empty result keeps the old path
+    };
+    int nt = std::min(std::max(1, s->replan_threads), n);
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) work(i);
+    } else {
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t)
+            th.emplace_back([&]() { for (int i; (i = next++) < n;) work(i); });
+        for (int i; (i = next++) < n;) work(i);
+        for (auto& t : th) t.join();
+    }
+    std::vector<int> ids, offs, cnts;
+    std::vector<float2> pts;
+    for (int i = 0; i < n; ++i) {
+        if (!cnt[i]) continue;
+        int id = rid[i];
+        ids.push_back(id);
+        offs.push_back((int)(s->path_used + pts.size()));
+        cnts.push_back(cnt[i]);
+        for (int q = 0; q < cnt[i]; ++q) pts.push_back(make_float2(res[i][2 * q], res[i][2 * q + 1]));
+        s->h_goal[id] = pts.back();
+        s->h_path_cnt[id] = cnt[i];
+    }
+    if (ids.empty()) return 0;
+    int rc = ensure_path_capacity(s, pts.size());
+    if (rc) return rc;
+    DBuf<int> d;
+    rc = d.alloc(3 * ids.size());
+    if (rc) return rc;
+    const size_t m = ids.size();
+    HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, pts.data(), pts.size() * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(d.p, ids.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(d.p + m, offs.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(d.p + 2 * m, cnts.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(launch_path_scatter((int)m, d.p, d.p + m, d.p + 2 * m, s->path_off.p, s->path_cnt.p, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    d.release();
+    s->path_used += pts.size();
+    return 0;
+}
+
 static int step_once(ecm_sim* s) {
     // UpdateMaxAgentIndex (Simulator.cpp:481-492)
     while (s->last_idx >= 0 && !s->h_active[s->last_idx]) --s->last_idx;
@@ -839,50 +937,77 @@ static int step_once(ecm_sim* s) {
             if (rc) return rc;
         }
     }
+    {
+        int stride = s->maxo + s->prm.k;
+        if (stride != s->q3d_stride) {
+            const size_t N = (size_t)s->prm.max_agents;
+            rc = s->q3d_agent.alloc(N);
+            if (!rc) rc = s->q3d_info.alloc(N);
+            if (!rc) rc = s->q3d_vel.alloc(N);
+            if (!rc) rc = s->q3d_con.alloc(N * (size_t)stride);
+            if (rc) return rc;
+            s->q3d_stride = stride;
+        }
+    }
     DevState d = make_state(s);
     HIPCHK(launch_fill_u32(s->counters.p, 0u, 2, s->stream));
+    // the 3D-LP queue restarts at the cumulative count
+    HIPCHK(launch_fill_u32(s->counters.p + 2, 0u, 1, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
     HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
     HIPCHK(launch_attr_knn(d, s->stream));
+    // replan requests are final once k_attr_knn is done: read them back
+    // and plan on the host while k_orca runs
+    const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
+    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_replan_ids, s->replan_list.p, peek * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipEventRecord(s->ev_replan, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, s->stream));
+    HIPCHK(launch_orca(d, s->maxo, 0u, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
+    HIPCHK(hipEventSynchronize(s->ev_replan));
+    const uint32_t nr = s->h_counters[1];
+    std::vector<int> rid(s->h_replan_ids, s->h_replan_ids + std::min<uint32_t>(nr, peek));
+    std::vector<float2> rpos(s->h_replan_pos, s->h_replan_pos + std::min<uint32_t>(nr, peek));
+    if (nr > (uint32_t)peek) {
+        rid.resize(nr);
+        rpos.resize(nr);
+        HIPCHK(hipMemcpyAsync(rid.data() + peek, s->replan_list.p + peek, (nr - peek) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipMemcpyAsync(rpos.data() + peek, s->replan_pos.p + peek, (nr - peek) * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    // order as the reference visits slots (Simulator.cpp:545)
+    {
+        std::vector<int> ord(nr);
+        for (uint32_t i = 0; i < nr; ++i) ord[i] = (int)i;
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return rid[a] < rid[b]; });
+        std::vector<int> r2(nr);
+        std::vector<float2> p2(nr);
+        for (uint32_t i = 0; i < nr; ++i) { r2[i] = rid[ord[i]]; p2[i] = rpos[ord[i]]; }
+        rid.swap(r2);
+        rpos.swap(p2);
+    }
+    s->h_replans = rid;
+    if (nr && s->replan_fn) {
+        rc = run_replans(s, rid, rpos);
+        if (rc) return rc;
+    }
     HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
     const int n_before = s->num_agents;
-    uint32_t nd = s->h_counters[0], nr = s->h_counters[1];
+    uint32_t nd = s->h_counters[0];
     if (nd) {
         std::vector<int> ids(nd);
         HIPCHK(hipMemcpy(ids.data(), s->destroyed_list.p, nd * sizeof(int), hipMemcpyDeviceToHost));
         std::sort(ids.begin(), ids.end());      // reference visits slots in ascending order
         for (int id : ids) mark_destroyed(s, id);
     }
-    s->h_replans.clear();
-    if (nr) {
-        s->h_replans.resize(nr);
-        HIPCHK(hipMemcpy(s->h_replans.data(), s->replan_list.p, nr * sizeof(int), hipMemcpyDeviceToHost));
-        std::sort(s->h_replans.begin(), s->h_replans.end());
-        if (s->replan_fn) {
-            std::vector<float2> p(1);
-            std::vector<float> buf(2 * 4096);
-            for (int id : s->h_replans) {
-                HIPCHK(hipMemcpy(p.data(), s->pos[s->cur].p + id, sizeof(float2), hipMemcpyDeviceToHost));
-                int np = 0;
-                float2 g = s->h_goal[id];
-                int r = s->replan_fn(s->replan_user, id, p[0].x, p[0].y, g.x, g.y, s->h_radius[id], buf.data(), 4096, &np);
-                // UpdatePath (Simulator.cpp:97-124); an empty result keeps the old path
-                if (r == 0 && np >= 1) {
-                    rc = ecm_sim_set_path(s, id, buf.data(), np);
-                    if (rc) return rc;
-                }
-            }
-        }
-    }
     s->stat[0] += 1;
     s->stat[1] += (uint64_t)n_before;
-    s->stat[2] = s->h_counters[2];
+    s->stat[2] += s->h_counters[2];
     s->stat[3] = s->h_counters[4];
     s->stat[4] += nr;
     s->stat[5] += nd;
@@ -956,8 +1081,10 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
     HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     const float* f = (const float*)src;
-    if (c == ECM_C_CLEARANCE) for (size_t i = 0; i < N; ++i) s->h_radius[i] = f[i];
-    if (c == ECM_C_PREF_SPEED) for (size_t i = 0; i < N; ++i) s->h_speed[i] = f[i];
+    if (c == ECM_C_CLEARANCE)
+        for (size_t i = 0; i < N; ++i) { s->h_radius[i] = f[i]; s->max_radius = std::max(s->max_radius, f[i]); }
+    if (c == ECM_C_PREF_SPEED)
+        for (size_t i = 0; i < N; ++i) { s->h_speed[i] = f[i]; s->max_pref_speed = std::max(s->max_pref_speed, f[i]); }
     if (c == ECM_C_ACTIVE) {
         // rebuild the pool: inactive slots, lowest on top
         const uint8_t* a = (const uint8_t*)src;

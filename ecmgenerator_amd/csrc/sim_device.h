@@ -48,8 +48,15 @@ struct DevState {
     // ---- events ----
     int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests
     int* replan_list;     uint32_t* replan_n;      // UpdatePath requests
-    uint32_t* stats;      // [0] 2D-LP failures (3D used), [1] agents stepped,
-                          // [2] locate fallbacks to the exact-y list, [3] obstacle overflow
+    float2* replan_pos;                            // step-start position of each request
+    uint32_t* stats;      // [0] 2D-LP failures (3D queue tail, cumulative), [1] unused,
+                          // [2] locate fallbacks to the exact-y list, [3] unused
+    // 3D-LP queue: agents whose 2D LP failed, with their constraints
+    int* q3d_agent;         // sorted position
+    int4* q3d_info;         // (n constraints, n obstacle constraints, failed index, 0)
+    float2* q3d_vel;        // 2D LP result at the failure
+    float4* q3d_con;        // [queue][con_stride] constraints (nx, ny, px, py)
+    int con_stride;         // MAXO + k
 
     // ---- obstacles (Environment::GetObstacles, edge per vertex in scan order) ----
     int n_obst_edges;

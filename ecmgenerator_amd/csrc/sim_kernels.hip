@@ -36,6 +36,14 @@ __device__ __forceinline__ uint32_t morton2(uint32_t cx, uint32_t cy) {
 }
 __device__ __forceinline__ uint32_t n_grid_keys(const GridDesc& g) { return 1u << (2 * g.bits); }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i -> XCD i % 8).
+// Remap so that XCD x processes the x-th contiguous eighth of the Morton
+// order: neighbouring agents then share one L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 // ------------------------------------------------------------ grid build
 __global__ void k_grid_count(DevState s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -237,90 +245,109 @@ __device__ __forceinline__ bool nb_better(float d, int id, float d2, int id2) {
     return d < d2 || (d == d2 && id < id2);
 }
 
+// Insert (dd, ii, ss) into the ascending register list (d, id, slot) of
+// length K; a predicated shift network, no dynamic indexing.
 template <int K>
-struct NbList {
-    float d[K];
-    int id[K];
-    int slot[K];
-    int n;
-    __device__ void init() {
+__device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], int (&sl)[K], float dd, int ii, int ss) {
 #pragma unroll
-        for (int i = 0; i < K; ++i) { d[i] = INFINITY; id[i] = INT_MAX; slot[i] = -1; }
-        n = 0;
+    for (int i = K - 1; i > 0; --i) {
+        bool before_prev = nb_better(dd, ii, d[i - 1], id[i - 1]);
+        bool before_cur = nb_better(dd, ii, d[i], id[i]);
+        d[i] = before_prev ? d[i - 1] : (before_cur ? dd : d[i]);
+        id[i] = before_prev ? id[i - 1] : (before_cur ? ii : id[i]);
+        sl[i] = before_prev ? sl[i - 1] : (before_cur ? ss : sl[i]);
     }
-    __device__ void insert(float dd, int ii, int ss) {
-        if (!nb_better(dd, ii, d[K - 1], id[K - 1])) return;
-#pragma unroll
-        for (int i = K - 1; i > 0; --i) {
-            bool before_prev = nb_better(dd, ii, d[i - 1], id[i - 1]);
-            bool before_cur = nb_better(dd, ii, d[i], id[i]);
-            if (before_prev) { d[i] = d[i - 1]; id[i] = id[i - 1]; slot[i] = slot[i - 1]; }
-            else if (before_cur) { d[i] = dd; id[i] = ii; slot[i] = ss; }
-        }
-        if (nb_better(dd, ii, d[0], id[0])) { d[0] = dd; id[0] = ii; slot[0] = ss; }
-        if (n < K) ++n;
-    }
-    __device__ float kth(int k) const {
-        float r = d[0];
-#pragma unroll
-        for (int i = 1; i < K; ++i) r = (i == k - 1) ? d[i] : r;
-        return r;
-    }
-};
-
-template <int K>
-__device__ __forceinline__ void scan_cell(const DevState& s, uint32_t m, V2 p, NbList<K>& L) {
-    uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
-    for (uint32_t j = a; j < b; ++j) {
-        float4 q = s.sorted_pr[j];
-        float dx = q.x - p.x, dy = q.y - p.y;
-        float d2 = dx * dx + dy * dy;
-        if (d2 > kEps) L.insert(d2, __float_as_int(q.w), (int)j);
-    }
+    bool first = nb_better(dd, ii, d[0], id[0]);
+    d[0] = first ? dd : d[0];
+    id[0] = first ? ii : id[0];
+    sl[0] = first ? ss : sl[0];
 }
 
-// Ring search over the agent grid; stops when the k-th candidate is closer
-// than every unvisited cell (with a 2 mm guard for rounding).
 template <int K>
-__device__ void knn_query(const DevState& s, V2 p, NbList<K>& L) {
+__device__ __forceinline__ float nb_kth(const float (&d)[K], int k) {
+    float r = d[0];
+#pragma unroll
+    for (int i = 1; i < K; ++i) r = (i == k - 1) ? d[i] : r;
+    return r;
+}
+
+// Exact k nearest by ring search over the agent grid.  Rings are walked as
+// one loop over their 8r perimeter cells; a cell is skipped when its box
+// (open toward the outside on the grid border, where clamped agents live)
+// is farther than the current k-th candidate, and the search stops when the
+// whole unvisited region is.  A 2 mm guard covers rounding.  The list is
+// ordered by (d2, id), so the result does not depend on visiting order.
+template <int K>
+__device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot)[K]) {
     const GridDesc& g = s.ag;
-    int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
-    int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
-    L.init();
+    const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
+    const int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
+    float d[K];
+    int id[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) { d[i] = INFINITY; id[i] = INT_MAX; out_slot[i] = -1; }
+    int n = 0;
+    const float guard = 2e-3f;
+    // a cell box at squared distance dd2 is skipped when
+    // sqrt(dd2) - guard > sqrt(kth)  <=>  dd2 > (sqrt(kth) + guard)^2
+    float prune2 = INFINITY;
     for (int r = 0;; ++r) {
-        int xl = cx - r, xh = cx + r, yl = cy - r, yh = cy + r;
-        if (r == 0) {
-            scan_cell<K>(s, morton2(cx, cy), p, L);
-        } else {
-            int xa = max(xl, 0), xb = min(xh, g.side - 1);
-            for (int x = xa; x <= xb; ++x) {
-                if (yl >= 0) scan_cell<K>(s, morton2(x, yl), p, L);
-                if (yh < g.side) scan_cell<K>(s, morton2(x, yh), p, L);
+        const int perim = r == 0 ? 1 : 8 * r;
+        for (int q = 0; q < perim; ++q) {
+            int x, y;
+            if (r == 0) { x = cx; y = cy; }
+            else if (q < 2 * r + 1) { x = cx - r + q; y = cy + r; }
+            else if (q < 4 * r + 2) { x = cx - r + (q - (2 * r + 1)); y = cy - r; }
+            else if (q < 6 * r + 1) { x = cx - r; y = cy - r + 1 + (q - (4 * r + 2)); }
+            else { x = cx + r; y = cy - r + 1 + (q - (6 * r + 1)); }
+            if (x < 0 || y < 0 || x >= g.side || y >= g.side) continue;
+            if (prune2 < INFINITY) {
+                // distance from p to the cell box, border cells open outward
+                float bx0 = x == 0 ? -INFINITY : g.x0 + (float)x * g.cell;
+                float bx1 = x == g.side - 1 ? INFINITY : g.x0 + (float)(x + 1) * g.cell;
+                float by0 = y == 0 ? -INFINITY : g.y0 + (float)y * g.cell;
+                float by1 = y == g.side - 1 ? INFINITY : g.y0 + (float)(y + 1) * g.cell;
+                float ddx = fmaxf(fmaxf(bx0 - p.x, p.x - bx1), 0.0f);
+                float ddy = fmaxf(fmaxf(by0 - p.y, p.y - by1), 0.0f);
+                if (ddx * ddx + ddy * ddy > prune2) continue;
             }
-            int ya = max(yl + 1, 0), yb = min(yh - 1, g.side - 1);
-            for (int y = ya; y <= yb; ++y) {
-                if (xl >= 0) scan_cell<K>(s, morton2(xl, y), p, L);
-                if (xh < g.side) scan_cell<K>(s, morton2(xh, y), p, L);
+            const uint32_t m = morton2((uint32_t)x, (uint32_t)y);
+            const uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
+            for (uint32_t j = a; j < b; ++j) {
+                float4 c = s.sorted_pr[j];
+                float dx = c.x - p.x, dy = c.y - p.y;
+                float d2 = dx * dx + dy * dy;
+                int cid = __float_as_int(c.w);
+                if (d2 > kEps && nb_better(d2, cid, d[K - 1], id[K - 1])) {
+                    nb_insert<K>(d, id, out_slot, d2, cid, (int)j);
+                    n = min(n + 1, K);
+                    if (n >= s.k) {
+                        float rr = __fsqrt_rn(nb_kth<K>(d, s.k)) * 1.0001f + guard;
+                        prune2 = rr * rr;
+                    }
+                }
             }
         }
+        const int xl = cx - r, xh = cx + r, yl = cy - r, yh = cy + r;
         if (xl <= 0 && yl <= 0 && xh >= g.side - 1 && yh >= g.side - 1) break;
-        if (L.n >= s.k) {
+        if (n >= s.k) {
             float b = INFINITY;
             if (xl > 0) b = fminf(b, p.x - (g.x0 + (float)xl * g.cell));
             if (yl > 0) b = fminf(b, p.y - (g.y0 + (float)yl * g.cell));
             if (xh < g.side - 1) b = fminf(b, (g.x0 + (float)(xh + 1) * g.cell) - p.x);
             if (yh < g.side - 1) b = fminf(b, (g.y0 + (float)(yh + 1) * g.cell) - p.y);
-            b -= 2e-3f;
-            if (b > 0.0f && b * b > L.kth(s.k)) break;
+            b -= guard;
+            if (b > 0.0f && b * b > nb_kth<K>(d, s.k)) break;
         }
     }
+    return min(n, s.k);
 }
 
 // --------------------------------------- attraction + steering + neighbours
 template <int K>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
     float4 pr = s.sorted_pr[t];
     const int id = __float_as_int(pr.w);
@@ -343,8 +370,10 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
         V2 ap = mk(0.0f, 0.0f);
         if (irm_attraction(s, pos, path, np, ap, fb)) attr = ap;
         else {
+            // UpdatePath(e, currentPosition, goal) with the step-start position
             uint32_t q = atomicAdd(s.replan_n, 1u);
             s.replan_list[q] = id;
+            s.replan_pos[q] = make_float2(pos.x, pos.y);
         }
     }
     s.attraction[id] = make_float2(attr.x, attr.y);
@@ -362,12 +391,11 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     s.pref_vel[id] = make_float2(des.x, des.y);
 
     // k nearest (replaces KDTree::KNearestAgents)
-    NbList<K> L;
-    knn_query<K>(s, pos, L);
-    int cnt = min(L.n, s.k);
+    int slot[K];
+    const int cnt = knn_query<K>(s, pos, slot);
 #pragma unroll
     for (int i = 0; i < K; ++i)
-        if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = L.slot[i];
+        if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
     s.nbr_cnt[t] = cnt;
 }
 
@@ -617,12 +645,23 @@ __device__ void lp3d(float4* lds, int tid, int n, int nObst, int proj_base, floa
     }
 }
 
+// Velocity + position systems and the force between them
+// (Simulator.cpp:592-635, 676-677).
+__device__ __forceinline__ void integrate(const DevState& s, int id, V2 position, V2 velocity, V2 o) {
+    const V2 F = mk(o.x - velocity.x, o.y - velocity.y);
+    const V2 nv = mk(velocity.x + F.x * s.mass_recip * s.dt, velocity.y + F.y * s.mass_recip * s.dt);
+    const V2 npn = mk(position.x + (nv.x * s.dt), position.y + (nv.y * s.dt));
+    s.pos_out[id] = make_float2(npn.x, npn.y);
+    s.vel_out[id] = make_float2(nv.x, nv.y);
+    if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
+}
+
 template <int K, int MAXO>
 __global__ __launch_bounds__(64) void k_orca(DevState s) {
-    __shared__ float4 lds[(MAXO + 2 * K) * 64];
+    __shared__ float4 lds[(MAXO + K) * 64];
     const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
     const int tid = threadIdx.x;
-    const int t = blockIdx.x * 64 + tid;
+    const int t = xcd_block() * 64 + tid;
     if (t >= (int)n_active) return;
     const float4 pr = s.sorted_pr[t];
     const float2 v0 = s.sorted_v[t];
@@ -680,18 +719,42 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
 
     V2 o;
     int fail = lp2d(lds, tid, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
-    bool used3d = fail < nc;
-    if (used3d) lp3d(lds, tid, nc, nObst, MAXO + K, maxSpeed, fail, o);
-    uint64_t bal = __ballot(used3d);
-    if ((tid & 63) == 0 && bal) atomicAdd(&s.stats[0], (uint32_t)__popcll(bal));
+    if (fail < nc) {
+        // RandomizedLP3D runs in k_lp3d on a compacted queue: park the
+        // constraints, the 2D result and the failing index
+        uint32_t q = atomicAdd(&s.stats[0], 1u);
+        s.q3d_agent[q] = t;
+        s.q3d_info[q] = make_int4(nc, nObst, fail, 0);
+        s.q3d_vel[q] = make_float2(o.x, o.y);
+        float4* dst = s.q3d_con + (size_t)q * s.con_stride;
+        for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
+        return;
+    }
+    integrate(s, id, position, velocity, o);
+}
 
-    // force, UpdateVelocitySystem, UpdatePositionSystem
-    const V2 F = mk(o.x - velocity.x, o.y - velocity.y);
-    const V2 nv = mk(velocity.x + F.x * s.mass_recip * s.dt, velocity.y + F.y * s.mass_recip * s.dt);
-    const V2 npn = mk(position.x + (nv.x * s.dt), position.y + (nv.y * s.dt));
-    s.pos_out[id] = make_float2(npn.x, npn.y);
-    s.vel_out[id] = make_float2(nv.x, nv.y);
-    if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
+// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed,
+// then integration.  Queue entries are in stats[0] - q3d_base order.
+template <int K, int MAXO>
+__global__ __launch_bounds__(64) void k_lp3d(DevState s, uint32_t q_base) {
+    __shared__ float4 lds[(MAXO + 2 * K) * 64];
+    const uint32_t nq = s.stats[0] - q_base;
+    const int tid = threadIdx.x;
+    const uint32_t i = blockIdx.x * 64 + tid;
+    if (i >= nq) return;
+    const uint32_t q = q_base + i;
+    const int t = s.q3d_agent[q];
+    const int4 info = s.q3d_info[q];
+    const float4* src = s.q3d_con + (size_t)q * s.con_stride;
+    for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
+    const float4 pr = s.sorted_pr[t];
+    const float2 v0 = s.sorted_v[t];
+    const int id = __float_as_int(pr.w);
+    const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
+    float2 ov = s.q3d_vel[q];
+    V2 o = mk(ov.x, ov.y);
+    lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
+    integrate(s, id, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
@@ -713,6 +776,7 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int n) {
 
 // ------------------------------------------------------------- launchers
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+static inline int round8(int n) { return (n + 7) & ~7; }
 
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) {
     const uint32_t G = 1u << (2 * s.ag.bits);
@@ -726,25 +790,33 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
 }
 
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, hipStream_t st) {
-    dim3 g(cdiv(s.n_slots, 64)), b(64);
-    if (maxo <= 16) hipLaunchKernelGGL((k_orca<K, 16>), g, b, 0, st, s);
-    else if (maxo <= 32) hipLaunchKernelGGL((k_orca<K, 32>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_orca<K, 64>), g, b, 0, st, s);
+static void launch_orca_k(const DevState& s, int maxo, uint32_t q_base, hipStream_t st) {
+    dim3 g(round8(cdiv(s.n_slots, 64))), b(64);
+    dim3 g3(cdiv(s.n_slots, 64));
+    if (maxo <= 16) {
+        hipLaunchKernelGGL((k_orca<K, 16>), g, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 16>), g3, b, 0, st, s, q_base);
+    } else if (maxo <= 32) {
+        hipLaunchKernelGGL((k_orca<K, 32>), g, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 32>), g3, b, 0, st, s, q_base);
+    } else {
+        hipLaunchKernelGGL((k_orca<K, 64>), g, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 64>), g3, b, 0, st, s, q_base);
+    }
 }
 
 hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
-    dim3 g(cdiv(s.n_slots, 256)), b(256);
+    dim3 g(round8(cdiv(s.n_slots, 256))), b(256);
     if (s.k <= 5) hipLaunchKernelGGL(k_attr_knn<5>, g, b, 0, st, s);
     else if (s.k <= 10) hipLaunchKernelGGL(k_attr_knn<10>, g, b, 0, st, s);
     else hipLaunchKernelGGL(k_attr_knn<16>, g, b, 0, st, s);
     return hipGetLastError();
 }
 
-hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, st);
-    else launch_orca_k<16>(s, maxo, st);
+hipError_t launch_orca(const DevState& s, int maxo, uint32_t q_base, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, q_base, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, q_base, st);
+    else launch_orca_k<16>(s, maxo, q_base, st);
     return hipGetLastError();
 }
 
@@ -780,6 +852,21 @@ __global__ void k_spawn_scatter(int n, const int* ids, const float2* start, cons
     active[id] = 1;
     path_off[id] = off[i];
     path_cnt[id] = cnt[i];
+}
+
+// UpdatePath results for a batch of agents (Simulator.cpp:114-123)
+__global__ void k_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    path_off[ids[i]] = off[i];
+    path_cnt[ids[i]] = cnt[i];
+}
+
+hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt,
+                               hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_path_scatter, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ids, off, cnt, path_off, path_cnt);
+    return hipGetLastError();
 }
 
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,

@@ -62,6 +62,7 @@ class Simulator {
         check(ecm_scene_ecm_export(m_Scene, vxy.data(), nullptr, tgt.data(), nullptr, pts.data()), "ecm_export");
         check(ecm_sim_set_ecm(m_Sim, ev, vxy.data(), ee, tgt.data(), pts.data()), "set_ecm");
         check(ecm_sim_set_replan_callback(m_Sim, ecm_scene_replan_callback(), m_Scene), "replan_callback");
+        check(ecm_sim_set_replan_threads(m_Sim, 8), "replan_threads");
         const size_t N = (size_t)m_MaxNumEntities;
         m_Positions.resize(N); m_Velocities.resize(N); m_PreferredVelocities.resize(N);
         m_AttractionPoints.resize(N); m_Clearances.resize(N); m_Active.resize(N);

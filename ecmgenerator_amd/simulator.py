@@ -9,6 +9,7 @@ is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -151,6 +152,8 @@ class Simulator:
         """UpdatePath on IRM failure (Simulator.cpp:579-587) through a scene's planner."""
         fn = self._L.ecm_scene_replan_callback()
         N.check(self._L.ecm_sim_set_replan_callback(self._h, fn, scene.handle))
+        # the scene planner is thread-safe (read-only ECM and index)
+        N.check(self._L.ecm_sim_set_replan_threads(self._h, max(1, min(16, os.cpu_count() or 1))))
         self._planner = scene
 
     # ---- agents ---------------------------------------------------------

@@ -94,6 +94,10 @@ int ecm_sim_set_path(ecm_sim* s, int id, const float* path_xy, int n_points);
 typedef int (*ecm_replan_fn)(void* user, int id, float x, float y, float goal_x, float goal_y,
                              float clearance, float* out_xy, int cap, int* out_n);
 int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user);
+/* Threads that may call the replan callback concurrently within one step
+ * (default 1).  Only raise it for a thread-safe callback, such as
+ * ecm_scene_replan_callback(). */
+int ecm_sim_set_replan_threads(ecm_sim* s, int n);
 int ecm_sim_step(ecm_sim* s, int n_steps);
 /* Wait for the handle's stream. */
 int ecm_sim_sync(ecm_sim* s);
