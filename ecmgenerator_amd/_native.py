@@ -65,7 +65,7 @@ SIGNATURES = {
     "ecm_sim_get_neighbors": (_I, [_P, _PI, _PI]),
     "ecm_sim_get_replans": (_I, [_P, _PI, _I, _PI]),
     "ecm_sim_read_paths": (_I, [_P, _PI, _PI, _PF, C.c_size_t, C.POINTER(C.c_size_t)]),
-    "ecm_sim_stats": (_I, [_P, C.POINTER(C.c_uint64)]),
+    "ecm_sim_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
     "ecm_sim_set_timing": (_I, [_P, _I]),
     "ecm_sim_last_timing": (_I, [_P, _PF]),
     "ecm_sim_snapshot_save": (_I, [_P, C.c_char_p]),

@@ -20,7 +20,7 @@
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, uint32_t q_base, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
@@ -143,6 +143,7 @@ struct ecm_sim {
 
     // 3D-LP queue (capacity max_agents)
     DBuf<int> q3d_agent;
+    DBuf<int> ovf_agent;
     DBuf<int4> q3d_info;
     DBuf<float2> q3d_vel;
     DBuf<float4> q3d_con;
@@ -174,7 +175,7 @@ struct ecm_sim {
     ecm_replan_fn replan_fn = nullptr;
     void* replan_user = nullptr;
     int replan_threads = 1;
-    uint64_t stat[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev_replan = nullptr;
@@ -215,7 +216,7 @@ static void free_all(ecm_sim* s) {
     s->key.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
     s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
-    s->q3d_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
+    s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     s->oe0.release(); s->oe1.release(); s->oflags.release();
     s->obucket_start.release(); s->obucket_edges.release();
     s->cellA.release(); s->cellB.release(); s->ebucket_start.release(); s->ebucket_cells.release();
@@ -845,6 +846,7 @@ static DevState make_state(ecm_sim* s) {
     d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
     d.replan_pos = s->replan_pos.p;
     d.stats = s->counters.p + 2;
+    d.ovf_agent = s->ovf_agent.p;
     d.q3d_agent = s->q3d_agent.p; d.q3d_info = s->q3d_info.p; d.q3d_vel = s->q3d_vel.p;
     d.q3d_con = s->q3d_con.p; d.con_stride = s->q3d_stride;
     d.n_obst_edges = s->n_obst_edges;
@@ -942,6 +944,7 @@ static int step_once(ecm_sim* s) {
         if (stride != s->q3d_stride) {
             const size_t N = (size_t)s->prm.max_agents;
             rc = s->q3d_agent.alloc(N);
+            if (!rc) rc = s->ovf_agent.alloc(N);
             if (!rc) rc = s->q3d_info.alloc(N);
             if (!rc) rc = s->q3d_vel.alloc(N);
             if (!rc) rc = s->q3d_con.alloc(N * (size_t)stride);
@@ -950,9 +953,8 @@ static int step_once(ecm_sim* s) {
         }
     }
     DevState d = make_state(s);
-    HIPCHK(launch_fill_u32(s->counters.p, 0u, 2, s->stream));
-    // the 3D-LP queue restarts at the cumulative count
-    HIPCHK(launch_fill_u32(s->counters.p + 2, 0u, 1, s->stream));
+    // destroyed, replan, 3D-LP queue and overflow queue counts restart
+    HIPCHK(launch_fill_u32(s->counters.p, 0u, 4, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
     HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
@@ -965,7 +967,7 @@ static int step_once(ecm_sim* s) {
     HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipEventRecord(s->ev_replan, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, 0u, s->stream));
+    HIPCHK(launch_orca(d, s->maxo, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
     HIPCHK(hipEventSynchronize(s->ev_replan));
     const uint32_t nr = s->h_counters[1];
@@ -1011,6 +1013,7 @@ static int step_once(ecm_sim* s) {
     s->stat[3] = s->h_counters[4];
     s->stat[4] += nr;
     s->stat[5] += nd;
+    s->stat[6] += s->h_counters[3];
     if (s->timing) {
         float a = 0, b = 0, c = 0;
         HIPCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
@@ -1166,9 +1169,9 @@ extern "C" int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n) {
     return 0;
 }
 
-extern "C" int ecm_sim_stats(ecm_sim* s, uint64_t* out6) {
-    if (!s || !out6) return fail(ECM_E_ARG, "null argument");
-    for (int i = 0; i < 6; ++i) out6[i] = s->stat[i];
+extern "C" int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n) {
+    if (!s || !out || n < 0) return fail(ECM_E_ARG, "null argument");
+    for (int i = 0; i < n && i < ECM_SIM_N_STATS; ++i) out[i] = s->stat[i];
     return 0;
 }
 

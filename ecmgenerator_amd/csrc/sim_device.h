@@ -49,8 +49,10 @@ struct DevState {
     int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests
     int* replan_list;     uint32_t* replan_n;      // UpdatePath requests
     float2* replan_pos;                            // step-start position of each request
-    uint32_t* stats;      // [0] 2D-LP failures (3D queue tail, cumulative), [1] unused,
-                          // [2] locate fallbacks to the exact-y list, [3] unused
+    uint32_t* stats;      // [0] 3D-LP queue length, [1] obstacle-overflow queue length
+                          // (both reset every step), [2] locate fallbacks to the
+                          // exact-y list (cumulative), [3] unused
+    int* ovf_agent;         // overflow queue: sorted positions
     // 3D-LP queue: agents whose 2D LP failed, with their constraints
     int* q3d_agent;         // sorted position
     int4* q3d_info;         // (n constraints, n obstacle constraints, failed index, 0)

@@ -8,10 +8,12 @@
 //      (Simulator.cpp:538-590, IRMPathFollower.cpp:14-115, ECM.cpp:20-96,
 //      ECMCellCollection.cpp:57-90), steering (Simulator.cpp:638-657) and the
 //      exact k-nearest query (replaces KDTree::KNearestAgents, KDTree.cpp:85-202).
-//   3. k_orca<K, MAXO>: obstacle query (Simulator.cpp:259-292), ORCA
-//      half-planes (ORCA.cpp:60-424) staged in LDS, 2D LP (ORCA.cpp:428-587),
-//      3D LP (ORCA.cpp:592-669), velocity and position integration
-//      (Simulator.cpp:592-635, 676-677) fused.
+//   3. k_orca<K>: obstacle query (Simulator.cpp:259-292), ORCA half-planes
+//      (ORCA.cpp:60-424) staged in LDS, 2D LP (ORCA.cpp:428-587), velocity
+//      and position integration (Simulator.cpp:592-635, 676-677) fused.
+//      Two compacted queues take the rare cases: k_orca_ovf (more obstacle
+//      constraints than the main tile holds) and k_lp3d (2D LP infeasible,
+//      RandomizedLP3D, ORCA.cpp:592-669).
 // One agent per lane; lanes are ordered along the Morton curve so a
 // wavefront covers a compact patch of the world.
 #include "ecm_geom.h"
@@ -242,29 +244,34 @@ __device__ bool irm_attraction(const DevState& s, V2 pos, const float2* path, in
 // ---------------------------------------------------------- exact k-NN
 // (d2, id) lexicographic order; d2 as KDTree.cpp:106-108.
 __device__ __forceinline__ bool nb_better(float d, int id, float d2, int id2) {
-    return d < d2 || (d == d2 && id < id2);
+    return (d < d2) | ((d == d2) & (id < id2));      // bitwise: no branches
 }
 
 // Insert (dd, ii, ss) into the ascending register list (d, id, slot) of
-// length K; a predicated shift network, no dynamic indexing.
+// length K: one comparison per entry (the flags are monotone along the
+// list), then a predicated shift; no dynamic indexing.
 template <int K>
 __device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], int (&sl)[K], float dd, int ii, int ss) {
+    bool b[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) b[i] = nb_better(dd, ii, d[i], id[i]);
 #pragma unroll
     for (int i = K - 1; i > 0; --i) {
-        bool before_prev = nb_better(dd, ii, d[i - 1], id[i - 1]);
-        bool before_cur = nb_better(dd, ii, d[i], id[i]);
-        d[i] = before_prev ? d[i - 1] : (before_cur ? dd : d[i]);
-        id[i] = before_prev ? id[i - 1] : (before_cur ? ii : id[i]);
-        sl[i] = before_prev ? sl[i - 1] : (before_cur ? ss : sl[i]);
+        d[i] = b[i - 1] ? d[i - 1] : (b[i] ? dd : d[i]);
+        id[i] = b[i - 1] ? id[i - 1] : (b[i] ? ii : id[i]);
+        sl[i] = b[i - 1] ? sl[i - 1] : (b[i] ? ss : sl[i]);
     }
-    bool first = nb_better(dd, ii, d[0], id[0]);
-    d[0] = first ? dd : d[0];
-    id[0] = first ? ii : id[0];
-    sl[0] = first ? ss : sl[0];
+    d[0] = b[0] ? dd : d[0];
+    id[0] = b[0] ? ii : id[0];
+    sl[0] = b[0] ? ss : sl[0];
 }
 
-template <int K>
+// k-th smallest distance of the list.  With EXACT (k == K) it is the last
+// register; otherwise a select chain, which the compiler may lower to an
+// indexed LDS copy of the list.
+template <int K, bool EXACT>
 __device__ __forceinline__ float nb_kth(const float (&d)[K], int k) {
+    if (EXACT) return d[K - 1];
     float r = d[0];
 #pragma unroll
     for (int i = 1; i < K; ++i) r = (i == k - 1) ? d[i] : r;
@@ -277,7 +284,7 @@ __device__ __forceinline__ float nb_kth(const float (&d)[K], int k) {
 // is farther than the current k-th candidate, and the search stops when the
 // whole unvisited region is.  A 2 mm guard covers rounding.  The list is
 // ordered by (d2, id), so the result does not depend on visiting order.
-template <int K>
+template <int K, bool EXACT>
 __device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot)[K]) {
     const GridDesc& g = s.ag;
     const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
@@ -322,7 +329,7 @@ __device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot
                     nb_insert<K>(d, id, out_slot, d2, cid, (int)j);
                     n = min(n + 1, K);
                     if (n >= s.k) {
-                        float rr = __fsqrt_rn(nb_kth<K>(d, s.k)) * 1.0001f + guard;
+                        float rr = __fsqrt_rn(nb_kth<K, EXACT>(d, s.k)) * 1.0001f + guard;
                         prune2 = rr * rr;
                     }
                 }
@@ -337,14 +344,14 @@ __device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot
             if (xh < g.side - 1) b = fminf(b, (g.x0 + (float)(xh + 1) * g.cell) - p.x);
             if (yh < g.side - 1) b = fminf(b, (g.y0 + (float)(yh + 1) * g.cell) - p.y);
             b -= guard;
-            if (b > 0.0f && b * b > nb_kth<K>(d, s.k)) break;
+            if (b > 0.0f && b * b > nb_kth<K, EXACT>(d, s.k)) break;
         }
     }
     return min(n, s.k);
 }
 
 // --------------------------------------- attraction + steering + neighbours
-template <int K>
+template <int K, bool EXACT>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
-    const int cnt = knn_query<K>(s, pos, slot);
+    const int cnt = knn_query<K, EXACT>(s, pos, slot);
 #pragma unroll
     for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
@@ -656,13 +663,14 @@ __device__ __forceinline__ void integrate(const DevState& s, int id, V2 position
     if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
 }
 
-template <int K, int MAXO>
-__global__ __launch_bounds__(64) void k_orca(DevState s) {
-    __shared__ float4 lds[(MAXO + K) * 64];
-    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
-    const int tid = threadIdx.x;
-    const int t = xcd_block() * 64 + tid;
-    if (t >= (int)n_active) return;
+// One agent of ObstacleAvoidanceForce (Simulator.cpp:659-686): obstacle
+// query, constraints into the LDS tile (OSLOTS obstacle + K agent slots),
+// RandomizedLP.  With OVERFLOW, an agent that needs more than OSLOTS
+// obstacle constraints is parked on the overflow queue untouched and
+// redone by k_orca_ovf with a full tile.  A failed 2D LP parks the agent on
+// the 3D-LP queue.
+template <int K, int OSLOTS, bool OVERFLOW>
+__device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds, int tid) {
     const float4 pr = s.sorted_pr[t];
     const float2 v0 = s.sorted_v[t];
     const int id = __float_as_int(pr.w);
@@ -698,6 +706,10 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
                 if (sq_dist(c, position) < range2) {
                     V2 cp, cn;
                     if (obstacle_constraint(s, e, position, velocity, rad, cp, cn)) {
+                        if (OVERFLOW && nc == OSLOTS) {
+                            s.ovf_agent[atomicAdd(&s.stats[1], 1u)] = t;
+                            return;
+                        }
                         con_st(lds, nc, tid, cp, cn);
                         ++nc;
                     }
@@ -733,28 +745,51 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
     integrate(s, id, position, velocity, o);
 }
 
-// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed,
-// then integration.  Queue entries are in stats[0] - q3d_base order.
-template <int K, int MAXO>
-__global__ __launch_bounds__(64) void k_lp3d(DevState s, uint32_t q_base) {
-    __shared__ float4 lds[(MAXO + 2 * K) * 64];
-    const uint32_t nq = s.stats[0] - q_base;
+// Main ORCA pass: one agent per lane in Morton order, a small constraint
+// tile (kObstSlots obstacle slots) so that more waves fit per CU.
+constexpr int kObstSlots = 6;
+template <int K>
+__global__ __launch_bounds__(64) void k_orca(DevState s) {
+    __shared__ float4 lds[(kObstSlots + K) * 64];
+    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
     const int tid = threadIdx.x;
-    const uint32_t i = blockIdx.x * 64 + tid;
-    if (i >= nq) return;
-    const uint32_t q = q_base + i;
-    const int t = s.q3d_agent[q];
-    const int4 info = s.q3d_info[q];
-    const float4* src = s.q3d_con + (size_t)q * s.con_stride;
-    for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
-    const float4 pr = s.sorted_pr[t];
-    const float2 v0 = s.sorted_v[t];
-    const int id = __float_as_int(pr.w);
-    const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
-    float2 ov = s.q3d_vel[q];
-    V2 o = mk(ov.x, ov.y);
-    lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
-    integrate(s, id, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
+    const int t = xcd_block() * 64 + tid;
+    if (t >= (int)n_active) return;
+    orca_agent<K, kObstSlots, true>(s, t, lds, tid);
+}
+
+// Agents with more than kObstSlots obstacle constraints, with a tile for
+// MAXO (>= the longest obstacle bucket list).  Fixed grid, strided loop.
+template <int K, int MAXO>
+__global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
+    __shared__ float4 lds[(MAXO + K) * 64];
+    const uint32_t nq = s.stats[1];
+    const int tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x * 64 + tid; i < nq; i += gridDim.x * 64)
+        orca_agent<K, MAXO, false>(s, s.ovf_agent[i], lds, tid);
+}
+
+// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed,
+// then integration.  Fixed grid, strided loop over the queue.
+template <int K, int MAXO>
+__global__ __launch_bounds__(64) void k_lp3d(DevState s) {
+    __shared__ float4 lds[(MAXO + 2 * K) * 64];
+    const uint32_t nq = s.stats[0];
+    const int tid = threadIdx.x;
+    for (uint32_t q = blockIdx.x * 64 + tid; q < nq; q += gridDim.x * 64) {
+        const int t = s.q3d_agent[q];
+        const int4 info = s.q3d_info[q];
+        const float4* src = s.q3d_con + (size_t)q * s.con_stride;
+        for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
+        const float4 pr = s.sorted_pr[t];
+        const float2 v0 = s.sorted_v[t];
+        const int id = __float_as_int(pr.w);
+        const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
+        float2 ov = s.q3d_vel[q];
+        V2 o = mk(ov.x, ov.y);
+        lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
+        integrate(s, id, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
+    }
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
@@ -789,34 +824,40 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
     return hipGetLastError();
 }
 
+// Queue kernels run on a fixed grid: their queues hold a few percent of
+// the agents, and a full-size grid would reserve their LDS for nothing.
+constexpr int kQueueBlocks = 1024;
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, uint32_t q_base, hipStream_t st) {
-    dim3 g(round8(cdiv(s.n_slots, 64))), b(64);
-    dim3 g3(cdiv(s.n_slots, 64));
+static void launch_orca_k(const DevState& s, int maxo, hipStream_t st) {
+    dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
+    hipLaunchKernelGGL((k_orca<K>), g, b, 0, st, s);
     if (maxo <= 16) {
-        hipLaunchKernelGGL((k_orca<K, 16>), g, b, 0, st, s);
-        hipLaunchKernelGGL((k_lp3d<K, 16>), g3, b, 0, st, s, q_base);
+        hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 16>), gq, b, 0, st, s);
     } else if (maxo <= 32) {
-        hipLaunchKernelGGL((k_orca<K, 32>), g, b, 0, st, s);
-        hipLaunchKernelGGL((k_lp3d<K, 32>), g3, b, 0, st, s, q_base);
+        hipLaunchKernelGGL((k_orca_ovf<K, 32>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 32>), gq, b, 0, st, s);
     } else {
-        hipLaunchKernelGGL((k_orca<K, 64>), g, b, 0, st, s);
-        hipLaunchKernelGGL((k_lp3d<K, 64>), g3, b, 0, st, s, q_base);
+        hipLaunchKernelGGL((k_orca_ovf<K, 64>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d<K, 64>), gq, b, 0, st, s);
     }
 }
 
 hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
     dim3 g(round8(cdiv(s.n_slots, 256))), b(256);
-    if (s.k <= 5) hipLaunchKernelGGL(k_attr_knn<5>, g, b, 0, st, s);
-    else if (s.k <= 10) hipLaunchKernelGGL(k_attr_knn<10>, g, b, 0, st, s);
-    else hipLaunchKernelGGL(k_attr_knn<16>, g, b, 0, st, s);
+    if (s.k == 5) hipLaunchKernelGGL((k_attr_knn<5, true>), g, b, 0, st, s);
+    else if (s.k == 10) hipLaunchKernelGGL((k_attr_knn<10, true>), g, b, 0, st, s);
+    else if (s.k < 5) hipLaunchKernelGGL((k_attr_knn<5, false>), g, b, 0, st, s);
+    else if (s.k < 10) hipLaunchKernelGGL((k_attr_knn<10, false>), g, b, 0, st, s);
+    else if (s.k == 16) hipLaunchKernelGGL((k_attr_knn<16, true>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_attr_knn<16, false>), g, b, 0, st, s);
     return hipGetLastError();
 }
 
-hipError_t launch_orca(const DevState& s, int maxo, uint32_t q_base, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, q_base, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, q_base, st);
-    else launch_orca_k<16>(s, maxo, q_base, st);
+hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, st);
+    else launch_orca_k<16>(s, maxo, st);
     return hipGetLastError();
 }
 

@@ -252,9 +252,10 @@ class Simulator:
         return buf[: n.value].copy()
 
     def stats(self) -> dict:
-        a = (C.c_uint64 * 6)()
-        N.check(self._L.ecm_sim_stats(self._h, a))
-        keys = ["steps", "agent_updates", "lp3d", "locate_fallback", "replans", "destroyed"]
+        keys = ["steps", "agent_updates", "lp3d", "locate_fallback", "replans", "destroyed",
+                "obst_overflow", "learned_tried", "learned_accepted"]
+        a = (C.c_uint64 * 10)()
+        N.check(self._L.ecm_sim_stats(self._h, a, 10))
         return {k: int(v) for k, v in zip(keys, a)}
 
     def set_timing(self, on: bool) -> None:

@@ -132,8 +132,12 @@ int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt);
 int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n);
 
 /* Counters since create: [0] steps, [1] agent updates, [2] 2D-LP failures
- * (3D LP run), [3] exact-y locate fallbacks, [4] replans, [5] destroyed. */
-int ecm_sim_stats(ecm_sim* s, uint64_t* out6);
+ * (3D LP run), [3] exact-y locate fallbacks, [4] replans, [5] destroyed,
+ * [6] agents with more obstacle constraints than the main ORCA tile holds,
+ * [7] agents the learned proposal was tried on, [8] of those, accepted by
+ * the verification, [9] reserved.  Copies min(n, ECM_SIM_N_STATS). */
+#define ECM_SIM_N_STATS 10
+int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n);
 /* Per-kernel device time of the last step in ms: [0] grid build,
  * [1] attraction + k-nearest, [2] ORCA + LP + integration.  Enable with
  * ecm_sim_set_timing(s, 1). */

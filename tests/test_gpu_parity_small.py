@@ -49,3 +49,35 @@ def test_parity_dense_crowd():
     p = Params(max_agents=2048, keep_orca_velocity=True)
     worst = run_parity(11, p, n_agents=1800, half=30.0, n_rects=4, steps=(0, 3, 8, 15))
     assert all(v == 0 for v in worst.values()), worst
+
+
+def test_parity_pillar_forest_obstacle_overflow():
+    """1 m pillars on a 3.5 m lattice: about 40% of the agents get more
+    obstacle constraints (up to 14) than the main ORCA tile holds, so the
+    overflow queue kernel runs; the result must still match the oracle bit
+    for bit."""
+    from oracle import oracle_py as O
+    from ecmgenerator_amd import scenarios as S
+    half = 20.0
+    box = (-half, -half, half, half)
+    rects = [(x, y, x + 1, y + 1) for x in np.arange(-15, 15, 3.5) for y in np.arange(-15, 15, 3.5)]
+    obs = S.rects_to_obstacles(rects)
+    ecm = S.empty_square_ecm(*box)
+    rng = np.random.default_rng(5)
+    agents = S.straight_agents(rng, box, 400, rects, margin=0.4)
+    p = Params(max_agents=512, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    assert (sim.SpawnAgents(agents) >= 0).sum() > 300
+    worst = {}
+    for it in range(6):
+        before = oracle_state(sim)
+        ora = O.step(p, before, obs, ecm)
+        sim.Update()
+        res = compare_step(sim, before, ora)
+        print(f"pillars step={it} {res} {sim.stats()}", flush=True)
+        for k, v in res.items():
+            worst[k] = max(worst.get(k, 0), v)
+    st = sim.stats()
+    sim.close()
+    assert st["obst_overflow"] > 0, st
+    assert all(v == 0 for v in worst.values()), worst
