@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for HBM counters)")
+    ap.add_argument("--learned", default=None,
+                    help="proposal-network weights (.npz) for the learned leg; default: the committed weights if present")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--theta", type=float, default=None, help="proposal logit threshold (default: from the weights file)")
     return ap.parse_args()
 
 
@@ -78,6 +82,14 @@ def main():
     for _ in range(args.warmup):
         sim.Update()
     sim.sync()
+    import tempfile
+    snap_dir = tempfile.mkdtemp(prefix="ecm_bench_")
+    snap = os.path.join(snap_dir, "state.bin")
+    sim.snapshot_save(snap)
+    # both legs time the same steps: the state just loaded, one untimed step
+    # (buffer re-creation after the load), then K timed steps
+    first_ms = _first_step_after_load(sim, snap)
+    print(f"[bench] plain: first step after load {first_ms:.2f} ms (untimed)", file=sys.stderr, flush=True)
     st0 = sim.stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -87,7 +99,7 @@ def main():
     st1 = sim.stats()
     # per-kernel HIP-event timing on the handle's stream, separate steps
     sim.set_timing(True)
-    kt = np.zeros(3)
+    kt = np.zeros(4)
     n_kt = max(3, min(args.steps, 10))
     for _ in range(n_kt):
         sim.Update()
@@ -97,7 +109,7 @@ def main():
     wall = t1 - t0
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / wall
-    kt_mean = kt / n_kt
+    kt_mean = np.delete(kt / n_kt, 2)          # no proposal phase in the plain step
     names = ["grid_build", "attraction_knn", "orca_lp_integrate"]
     dom = int(np.argmax(kt_mean))
     n_per_step = updates / args.steps
@@ -112,6 +124,15 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": names[dom],
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
+    learned = None
+    wpath = args.learned or _default_weights()
+    if not args.no_learned and wpath and os.path.exists(wpath):
+        learned = learned_leg(sim, wl, snap, wpath, args)
+    try:
+        os.remove(snap)
+        os.rmdir(snap_dir)
+    except OSError:
+        pass
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb)
     cpu = None
@@ -135,9 +156,82 @@ def main():
                    "mean_path_points": round(mean_path, 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned": learned,
         "stats": {k: st1[k] - st0[k] for k in st1},
     }
     print(json.dumps(line), flush=True)
+
+
+def _first_step_after_load(sim, snap):
+    sim.snapshot_load(snap)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.Update()
+    sim.sync()
+    return 1000.0 * (time.perf_counter() - t0)
+
+
+def _default_weights():
+    from ecmgenerator_amd import learned as Lm
+    return Lm.DEFAULT_WEIGHTS
+
+
+def learned_leg(sim, wl, snap, wpath, args):
+    """The learned-proposal step over the same K steps from the same state:
+    throughput, acceptance, and the largest velocity difference to the plain
+    step on the first step (implied ORCA-velocity difference, m/s)."""
+    from ecmgenerator_amd.learned import ProposalNet
+    with np.load(wpath, allow_pickle=False) as z:
+        params = z["params"]
+        meta = json.loads(str(z["meta"])) if "meta" in z else {}
+    theta = args.theta if args.theta is not None else float(meta.get("theta", 0.0))
+    net = ProposalNet(params=params)
+    # first step from the snapshot, plain then learned, for the velocity check
+    sim.snapshot_load(snap)
+    act = sim.state()["active"].astype(bool)
+    sim.Update()
+    a = sim.state()
+    sim.snapshot_load(snap)
+    sim.set_learned(net, theta)
+    sim.Update()
+    b = sim.state()
+    alive = act & a["active"].astype(bool) & b["active"].astype(bool)
+    k_vel = wl.params.dt / wl.params.mass            # v_new - v_old = (v_orca - v_old) * dt / mass
+    max_dv = float(np.abs(a["velocity"][alive] - b["velocity"][alive]).max() / k_vel) if alive.any() else 0.0
+    max_dp = float(np.abs(a["position"][act] - b["position"][act]).max()) if act.any() else 0.0
+    # timed: the same K steps as the plain leg
+    first_ms = _first_step_after_load(sim, snap)
+    print(f"[bench] learned: first step after load {first_ms:.2f} ms (untimed)", file=sys.stderr, flush=True)
+    st0 = sim.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.Update()
+    sim.sync()
+    t1 = time.perf_counter()
+    st1 = sim.stats()
+    sim.set_timing(True)
+    kt = np.zeros(4)
+    n_kt = max(3, min(args.steps, 10))
+    for _ in range(n_kt):
+        sim.Update()
+        kt += np.array(sim.last_timing())
+    sim.set_timing(False)
+    sim.set_learned(None)
+    net.close()
+    upd = st1["agent_updates"] - st0["agent_updates"]
+    tried = st1["learned_tried"] - st0["learned_tried"]
+    acc = st1["learned_accepted"] - st0["learned_accepted"]
+    kt /= n_kt
+    return {"value": round(upd / (t1 - t0), 1), "unit": "agent-updates/s",
+            "ms_per_step": round(1000.0 * (t1 - t0) / args.steps, 4),
+            "acceptance": round(acc / max(tried, 1), 4), "tried": tried, "accepted": acc,
+            "fallback_queue": st1["lp3d"] - st0["lp3d"],
+            "max_dv_vs_plain": max_dv, "max_dp_vs_plain": max_dp, "tolerance_dv": 1e-4,
+            "theta": theta, "weights": os.path.relpath(wpath, ROOT),
+            "kernel_ms": {"grid_build": round(float(kt[0]), 4), "attraction_knn": round(float(kt[1]), 4),
+                          "features_mlp": round(float(kt[2]), 4), "orca_verify_integrate": round(float(kt[3]), 4)},
+            # network FLOP rate over the whole proposal phase (features + MLP)
+            "proposal_tflops": round(85248.0 * (upd / args.steps) / (kt[2] * 1e-3) / 1e12, 2) if kt[2] > 0 else None}
 
 
 def cpu_baseline(sim, wl, n_sample):

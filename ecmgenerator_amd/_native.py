@@ -1,4 +1,5 @@
-"""ctypes binding of libecmcrowd.so (include/ecm_sim.h, include/ecm_scene.h).
+"""ctypes binding of libecmcrowd.so (include/ecm_sim.h, include/ecm_scene.h,
+include/ecm_mlp.h).
 
 The library is built in-tree by ``python -m ecmgenerator_amd.build``.  Loading
 fails loudly when it is missing: there is no Python or CPU fallback for the
@@ -67,7 +68,7 @@ SIGNATURES = {
     "ecm_sim_read_paths": (_I, [_P, _PI, _PI, _PF, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ecm_sim_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
     "ecm_sim_set_timing": (_I, [_P, _I]),
-    "ecm_sim_last_timing": (_I, [_P, _PF]),
+    "ecm_sim_last_timing": (_I, [_P, _PF, _I]),
     "ecm_sim_snapshot_save": (_I, [_P, C.c_char_p]),
     "ecm_sim_snapshot_load": (_I, [_P, C.c_char_p]),
     "ecm_last_error": (C.c_char_p, []),
@@ -90,6 +91,21 @@ SIGNATURES = {
     "ecm_scenario_counts": (_I, [_P, _PI, C.POINTER(C.c_longlong)]),
     "ecm_scenario_export": (_I, [_P, _PF, _PF, _PF, _PF, _PI, _PI, _PF]),
     "ecm_scenario_destroy": (None, [_P]),
+    # ecm_mlp.h
+    "ecm_mlp_n_params": (_I, []),
+    "ecm_mlp_create": (_I, [C.c_uint64, C.POINTER(_P)]),
+    "ecm_mlp_destroy": (_I, [_P]),
+    "ecm_mlp_get_params": (_I, [_P, _PF, _I]),
+    "ecm_mlp_set_params": (_I, [_P, _PF, _I]),
+    "ecm_mlp_dataset_reserve": (_I, [_P, C.c_int64]),
+    "ecm_mlp_dataset_size": (_I, [_P, C.POINTER(C.c_int64)]),
+    "ecm_mlp_dataset_clear": (_I, [_P]),
+    "ecm_mlp_dataset_read": (_I, [_P, C.c_int64, _I, _PF, C.POINTER(C.c_uint32), _PF]),
+    "ecm_sim_log_step": (_I, [_P, _P, _PI]),
+    "ecm_mlp_train": (_I, [_P, _I, _I, _F, _F, _F, C.c_uint64, _PF]),
+    "ecm_mlp_forward": (_I, [_P, _PF, _I, _PF]),
+    "ecm_mlp_loss_grad": (_I, [_P, _PF, C.POINTER(C.c_uint32), _PF, _I, _F, _F, _PF, _PF]),
+    "ecm_sim_set_learned": (_I, [_P, _P, _F]),
 }
 
 

@@ -37,6 +37,8 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
 SOURCES = [
     os.path.join(CSRC, "sim_kernels.hip"),
     os.path.join(CSRC, "sim_api.cpp"),
+    os.path.join(CSRC, "mlp_kernels.hip"),
+    os.path.join(CSRC, "mlp_api.cpp"),
     os.path.join(HOST, "scene_ecm.cpp"),
     os.path.join(HOST, "scene_plan.cpp"),
     os.path.join(HOST, "scene_api.cpp"),

@@ -3,6 +3,8 @@
 // loop.  Compiled by hipcc for gfx950; the kernels live in sim_kernels.hip.
 #include "../../include/ecm_sim.h"
 #include "ecm_geom.h"
+#include "host_util.h"
+#include "mlp_host.h"
 #include "sim_device.h"
 
 #include <hip/hip_runtime.h>
@@ -20,7 +22,8 @@
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, hipStream_t st);
+hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
@@ -34,7 +37,7 @@ hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int*
 using namespace ecm;
 
 static thread_local std::string g_err;
-static int fail(int code, const char* fmt, ...) {
+int ecm::fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -43,11 +46,6 @@ static int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-#define HIPCHK(x)                                                                        \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess) return fail(ECM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
-    } while (0)
 
 extern "C" const char* ecm_last_error(void) { return g_err.c_str(); }
 
@@ -60,27 +58,6 @@ extern "C" int ecm_device_available(void) {
     if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
     return std::strncmp(pr.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
 }
-
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    int alloc(size_t count) {
-        release();
-        if (count == 0) count = 1;
-        if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            return fail(ECM_E_HIP, "hipMalloc of %zu bytes failed", count * sizeof(T));
-        }
-        n = count;
-        return 0;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
 
 static GridDesc make_grid(float x0, float y0, float x1, float y1, float cell, int max_side) {
     GridDesc g;
@@ -126,7 +103,9 @@ struct ecm_sim {
     static const int kReplanPeek = 1024;     // replan requests read back while k_orca runs
     int* h_replan_ids = nullptr;             // pinned [kReplanPeek]
     float2* h_replan_pos = nullptr;          // pinned [kReplanPeek]
-    DBuf<uint32_t> counters;     // [0] destroyed_n [1] replan_n [2..5] stats
+    // [0] destroyed_n [1] replan_n [2..7] stats [8..263] per-wavefront slots
+    static const int kNCounters = 8 + 256;
+    DBuf<uint32_t> counters;
     uint32_t* h_counters = nullptr;
 
     // obstacles
@@ -177,9 +156,16 @@ struct ecm_sim {
     int replan_threads = 1;
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev_replan = nullptr;
-    float last_ms[3] = {0, 0, 0};
+    float last_ms[4] = {0, 0, 0, 0};
+
+    // learned proposal (features + network + verification) and sample log
+    ecm_mlp* mlp = nullptr;
+    float mlp_theta = 0.0f;
+    DBuf<uint4> feat;
+    DBuf<uint32_t> mlp_mask;
+    ecm_mlp* log_mlp = nullptr;     // set for the duration of ecm_sim_log_step
     std::vector<int> h_replans;
 };
 
@@ -228,6 +214,8 @@ static void free_all(ecm_sim* s) {
     s->h_replan_ids = nullptr;
     s->h_replan_pos = nullptr;
     s->replan_pos.release();
+    s->feat.release();
+    s->mlp_mask.release();
     for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (s->ev_replan) (void)hipEventDestroy(s->ev_replan);
     s->ev_replan = nullptr;
@@ -282,15 +270,15 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->nbr_cnt.alloc(N);
     if (!rc) rc |= s->destroyed_list.alloc(N);
     if (!rc) rc |= s->replan_list.alloc(N);
-    if (!rc) rc |= s->counters.alloc(8);
+    if (!rc) rc |= s->counters.alloc(ecm_sim::kNCounters);
     if (!rc) rc |= s->replan_pos.alloc(N);
-    if (!rc && hipHostMalloc((void**)&s->h_counters, 8 * sizeof(uint32_t)) != hipSuccess)
+    if (!rc && hipHostMalloc((void**)&s->h_counters, ecm_sim::kNCounters * sizeof(uint32_t)) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipHostMalloc((void**)&s->h_replan_ids, ecm_sim::kReplanPeek * sizeof(int)) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipHostMalloc((void**)&s->h_replan_pos, ecm_sim::kReplanPeek * sizeof(float2)) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
-    for (int i = 0; i < 4 && !rc; ++i)
+    for (int i = 0; i < 5 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc && hipEventCreateWithFlags(&s->ev_replan, hipEventDisableTiming) != hipSuccess)
         rc = fail(ECM_E_HIP, "event creation failed");
@@ -366,7 +354,6 @@ static double seg_box_dist2(double ax, double ay, double bx, double by, double x
 // Bucket every obstacle edge into the cells within `range` of it (capsule
 // test), lists ascending by edge id = (obstacle, vertex) scan order.
 static int build_obstacle_buckets(ecm_sim* s, float range) {
-    const int ne = s->n_obst_edges;
     const float margin = 0.05f;
     const double R = (double)range + margin;
     float cell = 2.0f;
@@ -436,6 +423,10 @@ extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* rin
     for (int r = 0; r < n_rings; ++r)
         if (ro[r + 1] - ro[r] < 2) return fail(ECM_E_ARG, "obstacle %d has fewer than 2 vertices (Environment.cpp:288)", r);
     const int nv = ro[n_rings];
+    // unchanged rings (a snapshot of the same scene): keep the buckets
+    if (s->obs_ring_off == ro && s->obs_xy.size() == 2 * (size_t)nv &&
+        (nv == 0 || std::memcmp(s->obs_xy.data(), xy, 2 * (size_t)nv * sizeof(float)) == 0))
+        return 0;
     s->obs_xy.assign(xy, xy + 2 * (size_t)nv);
     s->obs_ring_off = ro;
     s->n_obst_edges = nv;
@@ -846,7 +837,19 @@ static DevState make_state(ecm_sim* s) {
     d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
     d.replan_pos = s->replan_pos.p;
     d.stats = s->counters.p + 2;
+    d.wstats = s->counters.p + 8;
     d.ovf_agent = s->ovf_agent.p;
+    d.feat = s->feat.p;
+    d.mlp_mask = s->mlp_mask.p;
+    if (s->log_mlp) {
+        d.log_x = reinterpret_cast<uint4*>(s->log_mlp->ds_x.p);
+        d.log_label = s->log_mlp->ds_label.p;
+        d.log_v = s->log_mlp->ds_v.p;
+        d.log_n = s->log_mlp->ds_count.p;
+        d.log_cap = (uint32_t)s->log_mlp->ds_cap;
+    } else {
+        d.log_x = nullptr; d.log_label = nullptr; d.log_v = nullptr; d.log_n = nullptr; d.log_cap = 0;
+    }
     d.q3d_agent = s->q3d_agent.p; d.q3d_info = s->q3d_info.p; d.q3d_vel = s->q3d_vel.p;
     d.q3d_con = s->q3d_con.p; d.con_stride = s->q3d_stride;
     d.n_obst_edges = s->n_obst_edges;
@@ -953,8 +956,8 @@ static int step_once(ecm_sim* s) {
         }
     }
     DevState d = make_state(s);
-    // destroyed, replan, 3D-LP queue and overflow queue counts restart
-    HIPCHK(launch_fill_u32(s->counters.p, 0u, 4, s->stream));
+    // destroyed / replan lists and every per-step counter restart
+    HIPCHK(launch_fill_u32(s->counters.p, 0u, ecm_sim::kNCounters, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
     HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
@@ -967,8 +970,18 @@ static int step_once(ecm_sim* s) {
     HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipEventRecord(s->ev_replan, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, s->stream));
+    int mode = s->log_mlp ? 1 : 0;
+    if (s->mlp && !s->log_mlp) {
+        // learned proposal: features, network, then the verifying ORCA pass
+        mode = 2;
+        HIPCHK(launch_features(d, s->stream));
+        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + ((size_t)1 << (2 * s->ag.bits)), 0, s->mlp->frags.p,
+                                     s->mlp->pbias.p, s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu,
+                                     s->stream));
+    }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
+    HIPCHK(launch_orca(d, s->maxo, mode, s->stream));
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
     HIPCHK(hipEventSynchronize(s->ev_replan));
     const uint32_t nr = s->h_counters[1];
     std::vector<int> rid(s->h_replan_ids, s->h_replan_ids + std::min<uint32_t>(nr, peek));
@@ -996,7 +1009,8 @@ static int step_once(ecm_sim* s) {
         rc = run_replans(s, rid, rpos);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, ecm_sim::kNCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
     const int n_before = s->num_agents;
@@ -1010,18 +1024,21 @@ static int step_once(ecm_sim* s) {
     s->stat[0] += 1;
     s->stat[1] += (uint64_t)n_before;
     s->stat[2] += s->h_counters[2];
-    s->stat[3] = s->h_counters[4];
+    s->stat[3] += s->h_counters[4];
     s->stat[4] += nr;
     s->stat[5] += nd;
     s->stat[6] += s->h_counters[3];
+    for (int i = 0; i < 64; ++i) {
+        s->stat[7] += s->h_counters[8 + 4 * i];
+        s->stat[8] += s->h_counters[8 + 4 * i + 1];
+        s->stat[9] += s->h_counters[8 + 4 * i + 2];
+    }
     if (s->timing) {
-        float a = 0, b = 0, c = 0;
-        HIPCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
-        HIPCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
-        HIPCHK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
-        s->last_ms[0] = a;
-        s->last_ms[1] = b;
-        s->last_ms[2] = c;
+        for (int i = 0; i < 4; ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+            s->last_ms[i] = ms;
+        }
     }
     return 0;
 }
@@ -1180,9 +1197,56 @@ extern "C" int ecm_sim_set_timing(ecm_sim* s, int on) {
     s->timing = on != 0;
     return 0;
 }
-extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out3) {
-    if (!s || !out3) return fail(ECM_E_ARG, "null argument");
-    for (int i = 0; i < 3; ++i) out3[i] = s->last_ms[i];
+extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out, int n) {
+    if (!s || !out || n < 0) return fail(ECM_E_ARG, "null argument");
+    for (int i = 0; i < n && i < 4; ++i) out[i] = s->last_ms[i];
+    return 0;
+}
+
+// ------------------------------------------------------ learned proposal
+static int ensure_learned_buffers(ecm_sim* s) {
+    const size_t N = (size_t)s->prm.max_agents;
+    if (s->feat.n < N * 8) {
+        int rc = s->feat.alloc(N * 8);
+        if (!rc) rc = s->mlp_mask.alloc(N);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int ecm_sim_set_learned(ecm_sim* s, ecm_mlp* m, float theta) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    if (!std::isfinite(theta)) return fail(ECM_E_ARG, "theta must be finite");
+    (void)hipSetDevice(s->dev);
+    if (m) {
+        if (m->dev != s->dev) return fail(ECM_E_ARG, "network and simulator on different devices");
+        int rc = ensure_learned_buffers(s);
+        if (rc) return rc;
+        rc = ecm_mlp_pack(m);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(m->stream));      // packed weights ready before the sim stream reads them
+    }
+    s->mlp = m;
+    s->mlp_theta = theta;
+    return 0;
+}
+
+extern "C" int ecm_sim_log_step(ecm_sim* s, ecm_mlp* m, int* n_logged) {
+    if (!s || !m) return fail(ECM_E_ARG, "null argument");
+    if (m->dev != s->dev) return fail(ECM_E_ARG, "network and simulator on different devices");
+    if (m->ds_cap < 1) return fail(ECM_E_STATE, "reserve dataset capacity first (ecm_mlp_dataset_reserve)");
+    (void)hipSetDevice(s->dev);
+    int64_t before = 0;
+    int rc = ecm_mlp_dataset_size(m, &before);
+    if (rc) return rc;
+    s->log_mlp = m;
+    rc = step_once(s);
+    s->log_mlp = nullptr;
+    if (rc) return rc;
+    int64_t after = 0;
+    rc = ecm_mlp_dataset_size(m, &after);
+    if (rc) return rc;
+    if (n_logged) *n_logged = (int)(after - before);
     return 0;
 }
 

@@ -49,9 +49,10 @@ struct DevState {
     int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests
     int* replan_list;     uint32_t* replan_n;      // UpdatePath requests
     float2* replan_pos;                            // step-start position of each request
-    uint32_t* stats;      // [0] 3D-LP queue length, [1] obstacle-overflow queue length
-                          // (both reset every step), [2] locate fallbacks to the
-                          // exact-y list (cumulative), [3] unused
+    uint32_t* stats;      // per step: [0] 3D-LP queue length, [1] obstacle-overflow queue
+                          // length, [2] locate fallbacks to the exact-y list, [3..5] unused
+    uint32_t* wstats;     // per step, [64][4] slots summed by the host: learned proposals
+                          // tried, accepted, constraints proposed, unused
     int* ovf_agent;         // overflow queue: sorted positions
     // 3D-LP queue: agents whose 2D LP failed, with their constraints
     int* q3d_agent;         // sorted position
@@ -59,6 +60,16 @@ struct DevState {
     float2* q3d_vel;        // 2D LP result at the failure
     float4* q3d_con;        // [queue][con_stride] constraints (nx, ny, px, py)
     int con_stride;         // MAXO + k
+
+    // ---- learned proposal ----
+    uint4* feat;                // [n_slots][8] BF16 network inputs by sorted position
+    const uint32_t* mlp_mask;   // [n_slots] predicted active slots by sorted position
+    // training-sample log (one plain step that records its LP solutions)
+    uint4* log_x;               // [log_cap][8] BF16 features
+    uint32_t* log_label;        // [log_cap] label words
+    float2* log_v;              // [log_cap] LP velocity
+    uint32_t* log_n;            // append counter
+    uint32_t log_cap;
 
     // ---- obstacles (Environment::GetObstacles, edge per vertex in scan order) ----
     int n_obst_edges;

@@ -17,6 +17,7 @@
 // One agent per lane; lanes are ordered along the Morton curve so a
 // wavefront covers a compact patch of the world.
 #include "ecm_geom.h"
+#include "mlp.h"
 #include "sim_device.h"
 
 #include <limits.h>
@@ -663,13 +664,190 @@ __device__ __forceinline__ void integrate(const DevState& s, int id, V2 position
     if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
 }
 
+// ----------------------------------------------- learned-proposal support
+// The 4 nearest obstacle edges in range (the ones the obstacle query keeps,
+// Simulator.cpp:259-292), by (squared distance to the closest point, edge
+// id), with the index of the constraint each emitted (-1: none).  They are
+// feature slots 10..13 of the proposal network.
+struct Near4 {
+    float d[4];
+    int e[4];
+    int c[4];
+};
+__device__ __forceinline__ void near4_init(Near4& T) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { T.d[i] = INFINITY; T.e[i] = INT_MAX; T.c[i] = -1; }
+}
+__device__ __forceinline__ void near4_insert(Near4& T, float dd, int e, int c) {
+    bool b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = (dd < T.d[i]) | ((dd == T.d[i]) & (e < T.e[i]));
+#pragma unroll
+    for (int i = 3; i > 0; --i) {
+        T.d[i] = b[i - 1] ? T.d[i - 1] : (b[i] ? dd : T.d[i]);
+        T.e[i] = b[i - 1] ? T.e[i - 1] : (b[i] ? e : T.e[i]);
+        T.c[i] = b[i - 1] ? T.c[i - 1] : (b[i] ? c : T.c[i]);
+    }
+    T.d[0] = b[0] ? dd : T.d[0];
+    T.e[0] = b[0] ? e : T.e[0];
+    T.c[0] = b[0] ? c : T.c[0];
+}
+
+__device__ __forceinline__ float feat_clamp(float x) {
+    return fminf(fmaxf(x, -mlp::kFeatClamp), mlp::kFeatClamp);
+}
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    const __bf16 x = (__bf16)a, y = (__bf16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+
+// The 64 BF16 features of one agent (layout in mlp.h) to out[0..7].
+template <int K>
+__device__ void agent_features(const DevState& s, int t, V2 position, V2 velocity, float rad, float2 pv, int cnt,
+                               const Near4& T, uint4* out) {
+    float f[mlp::kInPad];
+#pragma unroll
+    for (int i = 0; i < mlp::kInPad; ++i) f[i] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < mlp::kAgentSlots && j < K; ++j) {
+        if (j < cnt) {
+            const int sj = s.nbr[(size_t)j * s.n_slots + t];
+            const float4 q = s.sorted_pr[sj];
+            const float2 qv = s.sorted_v[sj];
+            f[4 * j + 0] = feat_clamp((q.x - position.x) * mlp::kPosScale);
+            f[4 * j + 1] = feat_clamp((q.y - position.y) * mlp::kPosScale);
+            f[4 * j + 2] = feat_clamp((qv.x - velocity.x) * mlp::kVelScale);
+            f[4 * j + 3] = feat_clamp((qv.y - velocity.y) * mlp::kVelScale);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < mlp::kObstFeat; ++i) {
+        if (T.e[i] != INT_MAX) {
+            const float4 a = s.oe0[T.e[i]];
+            f[40 + 4 * i + 0] = feat_clamp((a.x - position.x) * mlp::kPosScale);
+            f[40 + 4 * i + 1] = feat_clamp((a.y - position.y) * mlp::kPosScale);
+            f[40 + 4 * i + 2] = feat_clamp((a.z - position.x) * mlp::kPosScale);
+            f[40 + 4 * i + 3] = feat_clamp((a.w - position.y) * mlp::kPosScale);
+        }
+    }
+    f[56] = feat_clamp(pv.x * mlp::kVelScale);
+    f[57] = feat_clamp(pv.y * mlp::kVelScale);
+    f[58] = feat_clamp(velocity.x * mlp::kVelScale);
+    f[59] = feat_clamp(velocity.y * mlp::kVelScale);
+    f[60] = feat_clamp(rad * mlp::kRadScale);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        out[c] = make_uint4(pack_bf16x2(f[8 * c + 0], f[8 * c + 1]), pack_bf16x2(f[8 * c + 2], f[8 * c + 3]),
+                            pack_bf16x2(f[8 * c + 4], f[8 * c + 5]), pack_bf16x2(f[8 * c + 6], f[8 * c + 7]));
+}
+
+// Obstacle query with the reference's filter (Simulator.cpp:266-287) over
+// the agent's bucket; `emit(e, dd)` runs for every edge it keeps, in scan
+// order, and returns false to stop the scan.
+template <typename F>
+__device__ __forceinline__ void obstacle_query(const DevState& s, V2 position, float range2, F&& emit) {
+    int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
+    int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
+    int b = by * s.og.side + bx;
+    int e0 = s.obucket_start[b], e1 = s.obucket_start[b + 1];
+    for (int q = e0; q < e1; ++q) {
+        int e = s.obucket_edges[q];
+        float4 a = s.oe0[e];
+        V2 L = mk(a.x, a.y), R = mk(a.z, a.w);
+        float sld = det(L - position, R - L);
+        float sqd = sld * sld / sq_dist(L, R);
+        if (sqd < range2 && sld < 0.0f) {
+            V2 c = closest_on_segment(position, L, R);
+            const float dd = sq_dist(c, position);
+            if (dd < range2)
+                if (!emit(e, dd)) return;
+        }
+    }
+}
+
+// A queue slot for every active lane of the wavefront, with one atomic.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr) {
+    const unsigned long long act = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)act) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(act));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+}
+
+// Count `pred` over the active lanes of the wavefront with one atomic.
+__device__ __forceinline__ void wave_count(uint32_t* ctr, bool pred) {
+    const unsigned long long b = __ballot(pred);
+    const unsigned long long act = __ballot(1);
+    if (b && (int)__lane_id() == __ffsll((long long)act) - 1) atomicAdd(ctr, (uint32_t)__popcll(b));
+}
+// Sum of v < 32 over the active lanes, bit by bit (one atomic per bit).
+__device__ __forceinline__ void wave_sum5(uint32_t* ctr, uint32_t v) {
+    const unsigned long long act = __ballot(1);
+    const bool leader = (int)__lane_id() == __ffsll((long long)act) - 1;
+    uint32_t total = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) total += (uint32_t)__popcll(__ballot((v >> b) & 1u)) << b;
+    if (leader && total) atomicAdd(ctr, total);
+}
+
+// RandomizedLP (ORCA.cpp:428-587, useDirOpt false) restricted to the
+// constraints in `mask`, in index order, with the same arithmetic as lp2d.
+// Returns false when the restricted problem is infeasible.
+__device__ bool lp2d_mask(const float4* lds, int tid, uint32_t mask, V2 opt, float maxSpeed, V2& out) {
+    if (len(opt) > maxSpeed) out = normalized(opt) * maxSpeed;
+    else out = opt;
+    uint32_t done = 0;
+    for (uint32_t m = mask; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        const Con h = con_ld(lds, i, tid);
+        if (!con_contains(h, out)) {
+            V2 dir = right(h.n);
+            float dpd = dot(dir, h.p);
+            float disc = dpd * dpd + maxSpeed * maxSpeed - dot(h.p, h.p);
+            if (disc <= 0.0f) return false;
+            float sq = sqrtf(disc);
+            float lft = -dpd - sq;
+            float rgt = -dpd + sq;
+            for (uint32_t d = done; d; d &= d - 1) {
+                const Con hj = con_ld(lds, __ffs(d) - 1, tid);
+                float den = det(right(h.n), right(hj.n));
+                float num = det(right(hj.n), h.p - hj.p);
+                if (fabsf(den) <= kEps) {
+                    if (num < 0.0f) return false;
+                    continue;
+                }
+                const float t = num / den;
+                if (den >= 0.0f) rgt = std_min(rgt, t);
+                else lft = std_max(lft, t);
+                if (lft > rgt) return false;
+            }
+            float t = dot(right(h.n), opt - h.p);
+            if (t < lft) out = h.p + right(h.n) * lft;
+            else if (t > rgt) out = h.p + right(h.n) * rgt;
+            else out = h.p + right(h.n) * t;
+        }
+        done |= 1u << i;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------- ORCA passes
+// Modes of the main pass: the plain step, the plain step that also logs a
+// training sample per agent, and the learned proposal with verification.
+enum OrcaMode { kPlain = 0, kLog = 1, kLearned = 2 };
+
+constexpr float kActiveTol = 1e-3f;   // a constraint is labelled active within 1 mm/s of the LP velocity
+
 // One agent of ObstacleAvoidanceForce (Simulator.cpp:659-686): obstacle
 // query, constraints into the LDS tile (OSLOTS obstacle + K agent slots),
 // RandomizedLP.  With OVERFLOW, an agent that needs more than OSLOTS
 // obstacle constraints is parked on the overflow queue untouched and
 // redone by k_orca_ovf with a full tile.  A failed 2D LP parks the agent on
-// the 3D-LP queue.
-template <int K, int OSLOTS, bool OVERFLOW>
+// the 3D-LP queue; so does a rejected learned proposal (failed index -1:
+// the queue kernel runs the full 2D LP first).
+template <int K, int OSLOTS, bool OVERFLOW, int MODE>
 __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds, int tid) {
     const float4 pr = s.sorted_pr[t];
     const float2 v0 = s.sorted_v[t];
@@ -688,34 +866,27 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
 
     // obstacle neighbours in scan order, constraints generated in place
     int nc = 0;
+    Near4 T;
+    if (MODE != kPlain) near4_init(T);
     if (s.n_obst_edges > 0) {
         const float range = s.time_horizon_obst * maxSpeed + rad;          // ORCA.cpp:27
-        const float range2 = range * range;
-        int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
-        int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
-        int b = by * s.og.side + bx;
-        int e0 = s.obucket_start[b], e1 = s.obucket_start[b + 1];
-        for (int q = e0; q < e1; ++q) {
-            int e = s.obucket_edges[q];
-            float4 a = s.oe0[e];
-            V2 L = mk(a.x, a.y), R = mk(a.z, a.w);
-            float sld = det(L - position, R - L);
-            float sqd = sld * sld / sq_dist(L, R);
-            if (sqd < range2 && sld < 0.0f) {
-                V2 c = closest_on_segment(position, L, R);
-                if (sq_dist(c, position) < range2) {
-                    V2 cp, cn;
-                    if (obstacle_constraint(s, e, position, velocity, rad, cp, cn)) {
-                        if (OVERFLOW && nc == OSLOTS) {
-                            s.ovf_agent[atomicAdd(&s.stats[1], 1u)] = t;
-                            return;
-                        }
-                        con_st(lds, nc, tid, cp, cn);
-                        ++nc;
-                    }
-                }
+        bool parked = false;
+        obstacle_query(s, position, range * range, [&](int e, float dd) {
+            V2 cp, cn;
+            const bool made = obstacle_constraint(s, e, position, velocity, rad, cp, cn);
+            if (made && OVERFLOW && nc == OSLOTS) {
+                s.ovf_agent[wave_alloc(&s.stats[1])] = t;
+                parked = true;
+                return false;
             }
-        }
+            if (MODE != kPlain) near4_insert(T, dd, e, made ? nc : -1);
+            if (made) {
+                con_st(lds, nc, tid, cp, cn);
+                ++nc;
+            }
+            return true;
+        });
+        if (parked) return;
     }
     const int nObst = nc;
     const int cnt = s.nbr_cnt[t];
@@ -730,11 +901,43 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     }
 
     V2 o;
+    if (MODE == kLearned) {
+        // proposal: predicted slots -> constraint indices
+        const uint32_t pm = s.mlp_mask[t];
+        uint32_t cm = 0;
+#pragma unroll
+        for (int j = 0; j < mlp::kAgentSlots && j < K; ++j)
+            if (j < cnt && ((pm >> j) & 1u)) cm |= 1u << (nObst + j);
+#pragma unroll
+        for (int i = 0; i < mlp::kObstFeat; ++i)
+            if (((pm >> (mlp::kAgentSlots + i)) & 1u) && T.c[i] >= 0) cm |= 1u << T.c[i];
+        // verification: optimum of the restricted LP, contained by the rest
+        bool ok = lp2d_mask(lds, tid, cm, mk(pv.x, pv.y), maxSpeed, o);
+        for (int c = 0; ok && c < nc; ++c)
+            if (!((cm >> c) & 1u) && !con_contains(con_ld(lds, c, tid), o)) ok = false;
+        // per-wavefront counters spread over 64 slots (one address each
+        // would serialize 15k atomics per launch)
+        uint32_t* ws = s.wstats + 4 * (blockIdx.x & 63);
+        wave_count(ws + 0, true);
+        wave_count(ws + 1, ok);
+        wave_sum5(ws + 2, (uint32_t)__popc(cm));
+        if (!ok) {
+            uint32_t q = wave_alloc(&s.stats[0]);
+            s.q3d_agent[q] = t;
+            s.q3d_info[q] = make_int4(nc, nObst, -1, 0);
+            float4* dst = s.q3d_con + (size_t)q * s.con_stride;
+            for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
+            return;
+        }
+        integrate(s, id, position, velocity, o);
+        return;
+    }
+
     int fail = lp2d(lds, tid, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
     if (fail < nc) {
         // RandomizedLP3D runs in k_lp3d on a compacted queue: park the
         // constraints, the 2D result and the failing index
-        uint32_t q = atomicAdd(&s.stats[0], 1u);
+        uint32_t q = wave_alloc(&s.stats[0]);
         s.q3d_agent[q] = t;
         s.q3d_info[q] = make_int4(nc, nObst, fail, 0);
         s.q3d_vel[q] = make_float2(o.x, o.y);
@@ -742,35 +945,66 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
         return;
     }
+    if (MODE == kLog) {
+        // training sample: features, active slots at the LP velocity, v
+        const uint32_t idx = wave_alloc(s.log_n);
+        if (idx < s.log_cap) {
+            agent_features<K>(s, t, position, velocity, rad, pv, cnt, T, s.log_x + (size_t)idx * 8);
+            uint32_t act = 0, valid = 0;
+            bool repr = true;
+            for (int c = 0; c < nc; ++c) {
+                const Con h = con_ld(lds, c, tid);
+                const bool a = det(right(h.n), h.p - o) >= -kActiveTol;
+                int slot = -1;
+                if (c >= nObst) {
+                    if (c - nObst < mlp::kAgentSlots) slot = c - nObst;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < mlp::kObstFeat; ++i)
+                        if (T.c[i] == c) slot = mlp::kAgentSlots + i;
+                }
+                if (slot >= 0) {
+                    valid |= 1u << slot;
+                    if (a) act |= 1u << slot;
+                } else if (a) {
+                    repr = false;
+                }
+            }
+            s.log_label[idx] = act | (valid << mlp::kLabelValidShift) | (repr ? mlp::kLabelRepresentable : 0u);
+            s.log_v[idx] = make_float2(o.x, o.y);
+        }
+    }
     integrate(s, id, position, velocity, o);
 }
 
 // Main ORCA pass: one agent per lane in Morton order, a small constraint
 // tile (kObstSlots obstacle slots) so that more waves fit per CU.
 constexpr int kObstSlots = 6;
-template <int K>
+template <int K, int MODE>
 __global__ __launch_bounds__(64) void k_orca(DevState s) {
     __shared__ float4 lds[(kObstSlots + K) * 64];
     const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
     const int tid = threadIdx.x;
     const int t = xcd_block() * 64 + tid;
     if (t >= (int)n_active) return;
-    orca_agent<K, kObstSlots, true>(s, t, lds, tid);
+    orca_agent<K, kObstSlots, true, MODE>(s, t, lds, tid);
 }
 
 // Agents with more than kObstSlots obstacle constraints, with a tile for
-// MAXO (>= the longest obstacle bucket list).  Fixed grid, strided loop.
+// MAXO (>= the longest obstacle bucket list), plain LP.  Fixed grid,
+// strided loop.
 template <int K, int MAXO>
 __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     __shared__ float4 lds[(MAXO + K) * 64];
     const uint32_t nq = s.stats[1];
     const int tid = threadIdx.x;
     for (uint32_t i = blockIdx.x * 64 + tid; i < nq; i += gridDim.x * 64)
-        orca_agent<K, MAXO, false>(s, s.ovf_agent[i], lds, tid);
+        orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, tid);
 }
 
-// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed,
-// then integration.  Fixed grid, strided loop over the queue.
+// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed
+// (after the full 2D LP for a rejected proposal), then integration.  Fixed
+// grid, strided loop over the queue.
 template <int K, int MAXO>
 __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     __shared__ float4 lds[(MAXO + 2 * K) * 64];
@@ -778,18 +1012,56 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     const int tid = threadIdx.x;
     for (uint32_t q = blockIdx.x * 64 + tid; q < nq; q += gridDim.x * 64) {
         const int t = s.q3d_agent[q];
-        const int4 info = s.q3d_info[q];
+        int4 info = s.q3d_info[q];
         const float4* src = s.q3d_con + (size_t)q * s.con_stride;
         for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
         const float4 pr = s.sorted_pr[t];
         const float2 v0 = s.sorted_v[t];
         const int id = __float_as_int(pr.w);
         const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
-        float2 ov = s.q3d_vel[q];
-        V2 o = mk(ov.x, ov.y);
-        lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
+        V2 o;
+        if (info.z < 0) {
+            const float2 pv = s.pref_vel[id];
+            info.z = lp2d(lds, tid, info.x, info.x, 0, mk(pv.x, pv.y), maxSpeed, false, o);
+        } else {
+            const float2 ov = s.q3d_vel[q];
+            o = mk(ov.x, ov.y);
+        }
+        if (info.z < info.x) lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
         integrate(s, id, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
     }
+}
+
+// Proposal network inputs of every sorted agent (zeros for agents
+// destroyed this step).
+template <int K>
+__global__ __launch_bounds__(256) void k_features(DevState s) {
+    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= (int)n_active) return;
+    const float4 pr = s.sorted_pr[t];
+    const float2 v0 = s.sorted_v[t];
+    const int id = __float_as_int(pr.w);
+    uint4* out = s.feat + (size_t)t * 8;
+    if (!s.active[id]) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) out[c] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    const V2 position = mk(pr.x, pr.y);
+    const V2 velocity = mk(v0.x, v0.y);
+    const float rad = pr.z;
+    const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
+    Near4 T;
+    near4_init(T);
+    if (s.n_obst_edges > 0) {
+        const float range = s.time_horizon_obst * maxSpeed + rad;
+        obstacle_query(s, position, range * range, [&](int e, float dd) {
+            near4_insert(T, dd, e, -1);
+            return true;
+        });
+    }
+    agent_features<K>(s, t, position, velocity, rad, s.pref_vel[id], s.nbr_cnt[t], T, out);
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
@@ -828,9 +1100,11 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
 // the agents, and a full-size grid would reserve their LDS for nothing.
 constexpr int kQueueBlocks = 1024;
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, hipStream_t st) {
+static void launch_orca_k(const DevState& s, int maxo, int mode, hipStream_t st) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
-    hipLaunchKernelGGL((k_orca<K>), g, b, 0, st, s);
+    if (mode == kLearned) hipLaunchKernelGGL((k_orca<K, kLearned>), g, b, 0, st, s);
+    else if (mode == kLog) hipLaunchKernelGGL((k_orca<K, kLog>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_orca<K, kPlain>), g, b, 0, st, s);
     if (maxo <= 16) {
         hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 16>), gq, b, 0, st, s);
@@ -854,10 +1128,20 @@ hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_orca(const DevState& s, int maxo, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, st);
-    else launch_orca_k<16>(s, maxo, st);
+// mode: 0 plain, 1 plain + sample log, 2 learned proposal (needs the
+// predicted masks of launch_features + the proposal network)
+hipError_t launch_orca(const DevState& s, int maxo, int mode, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, st);
+    else launch_orca_k<16>(s, maxo, mode, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const DevState& s, hipStream_t st) {
+    dim3 g(round8(cdiv(s.n_slots, 256))), b(256);
+    if (s.k <= 5) hipLaunchKernelGGL((k_features<5>), g, b, 0, st, s);
+    else if (s.k <= 10) hipLaunchKernelGGL((k_features<10>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_features<16>), g, b, 0, st, s);
     return hipGetLastError();
 }
 

@@ -253,7 +253,7 @@ class Simulator:
 
     def stats(self) -> dict:
         keys = ["steps", "agent_updates", "lp3d", "locate_fallback", "replans", "destroyed",
-                "obst_overflow", "learned_tried", "learned_accepted"]
+                "obst_overflow", "learned_tried", "learned_accepted", "learned_proposed"]
         a = (C.c_uint64 * 10)()
         N.check(self._L.ecm_sim_stats(self._h, a, 10))
         return {k: int(v) for k, v in zip(keys, a)}
@@ -262,9 +262,23 @@ class Simulator:
         N.check(self._L.ecm_sim_set_timing(self._h, int(on)))
 
     def last_timing(self):
-        a = (C.c_float * 3)()
-        N.check(self._L.ecm_sim_last_timing(self._h, a))
+        """[grid build, attraction + k-nearest, learned proposal, ORCA + LP +
+        integration] device ms of the last step (with set_timing(True))."""
+        a = (C.c_float * 4)()
+        N.check(self._L.ecm_sim_last_timing(self._h, a, 4))
         return list(a)
+
+    def set_learned(self, net, theta: float = 0.0) -> None:
+        """Use the learned ORCA proposal of `net` (a learned.ProposalNet), or
+        the plain LP with net=None."""
+        N.check(self._L.ecm_sim_set_learned(self._h, net.handle if net is not None else None, float(theta)))
+        self._learned = net
+
+    def log_step(self, net) -> int:
+        """One plain step that appends training samples to `net`'s dataset."""
+        n = C.c_int()
+        N.check(self._L.ecm_sim_log_step(self._h, net.handle, C.byref(n)))
+        return n.value
 
     def snapshot_save(self, path: str) -> None:
         N.check(self._L.ecm_sim_snapshot_save(self._h, path.encode()))

@@ -135,14 +135,16 @@ int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n);
  * (3D LP run), [3] exact-y locate fallbacks, [4] replans, [5] destroyed,
  * [6] agents with more obstacle constraints than the main ORCA tile holds,
  * [7] agents the learned proposal was tried on, [8] of those, accepted by
- * the verification, [9] reserved.  Copies min(n, ECM_SIM_N_STATS). */
+ * the verification, [9] constraints proposed over the tried agents.
+ * Copies min(n, ECM_SIM_N_STATS). */
 #define ECM_SIM_N_STATS 10
 int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n);
-/* Per-kernel device time of the last step in ms: [0] grid build,
- * [1] attraction + k-nearest, [2] ORCA + LP + integration.  Enable with
- * ecm_sim_set_timing(s, 1). */
+/* Device time of the phases of the last step in ms (HIP events on the
+ * handle's stream): [0] grid build, [1] attraction + k-nearest, [2] learned
+ * proposal (features + network; 0 without one), [3] ORCA + LP +
+ * integration.  Enable with ecm_sim_set_timing(s, 1); copies min(n, 4). */
 int ecm_sim_set_timing(ecm_sim* s, int on);
-int ecm_sim_last_timing(ecm_sim* s, float* out3);
+int ecm_sim_last_timing(ecm_sim* s, float* out, int n);
 
 /* Snapshot of the whole state (SoA + paths + params) to / from a file. */
 int ecm_sim_snapshot_save(ecm_sim* s, const char* path);

@@ -393,9 +393,21 @@ static inline void push(con* cs, int* n, int cap, v2 p, v2 nrm) {
 }
 
 /* ORCA::GenerateConstraints (ORCA.cpp:60-424) */
+static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, const obst_soa* s,
+                              int e, const int* obst, int n_obst_nb, const int* nbr, int n_nbr,
+                              con* cs, int cap, int* n_obst_out, int* con_start);
 static int gen_constraints(const oracle_params* prm, const oracle_state* st, const obst_soa* s,
                            int e, const int* obst, int n_obst_nb, const int* nbr, int n_nbr,
                            con* cs, int cap, int* n_obst_out) {
+    return gen_constraints_ex(prm, st, s, e, obst, n_obst_nb, nbr, n_nbr, cs, cap, n_obst_out, NULL);
+}
+
+/* con_start (optional, n_obst_nb + 1 entries): constraint count before each
+ * obstacle edge is processed, and after the last, so that edge i emitted
+ * constraint con_start[i] iff con_start[i + 1] > con_start[i]. */
+static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, const obst_soa* s,
+                              int e, const int* obst, int n_obst_nb, const int* nbr, int n_nbr,
+                              con* cs, int cap, int* n_obst_out, int* con_start) {
     int nc = 0;
     v2 position = V(st->pos[2 * e], st->pos[2 * e + 1]);
     v2 velocity = V(st->vel[2 * e], st->vel[2 * e + 1]);
@@ -403,6 +415,7 @@ static int gen_constraints(const oracle_params* prm, const oracle_state* st, con
     const float inf = INFINITY;
 
     for (int i = 0; i < n_obst_nb; ++i) {                   /* :70-333 */
+        if (con_start) con_start[i] = nc;
         int L = obst[i];
         int R = s->next[L];
         v2 rp1 = vsub(s->p[L], position);
@@ -514,6 +527,7 @@ static int gen_constraints(const oracle_params* prm, const oracle_state* st, con
         }
     }
     *n_obst_out = nc;                                        /* :335 */
+    if (con_start) con_start[n_obst_nb] = nc;
 
     const float T = prm->time_horizon;
     const float dt = prm->dt;
@@ -919,6 +933,98 @@ int oracle_constraints(const oracle_params* prm, const oracle_state* st, const o
         out[4 * i] = cs[i].n.x; out[4 * i + 1] = cs[i].n.y;
         out[4 * i + 2] = cs[i].p.x; out[4 * i + 3] = cs[i].p.y;
     }
+    obst_free(&s);
+    return nc;
+}
+
+/* ------------------------------------------- learned-proposal samples */
+/* Inputs and label of the proposal network for one agent (the layout the
+ * library's step logs; no reference counterpart):
+ *   feat[64]: neighbour j < 10 relative position * 0.25 and velocity * 0.5
+ *   (j over nbrs in the order given), endpoints of the 4 nearest obstacle
+ *   edges kept by FindNearestObstacles (by squared distance to the closest
+ *   point, then edge index) relative to the agent * 0.25, preferred
+ *   velocity * 0.5, velocity * 0.5, radius * 4, each clamped to +-8;
+ *   *label: bit s (s < 14) slot s active at the LP velocity v (within
+ *   1e-3), bit 16 + s slot s holds a constraint, bit 30 every active
+ *   constraint is behind a slot. */
+int oracle_proposal_sample(const oracle_params* prm, const oracle_state* st, const oracle_obstacles* obs,
+                           int agent, const int* nbrs, int n_nbrs, const float* pref_vel, const float* v,
+                           float* feat, unsigned* label) {
+    obst_soa s;
+    memset(&s, 0, sizeof(s));
+    if (obs && obs->n_rings > 0) { if (obst_build(obs, &s) != 0) { obst_free(&s); return -2; } }
+    const float maxSpeed = prm->max_speed_from_pref ? st->pref_speed[agent] : prm->max_speed;
+    const float range = prm->time_horizon_obst * maxSpeed + st->radius[agent];
+    const v2 pos = V(st->pos[2 * agent], st->pos[2 * agent + 1]);
+    const v2 vel = V(st->vel[2 * agent], st->vel[2 * agent + 1]);
+    int obl[MAXC] = {0};
+    int nob = s.nv > 0 ? nearest_obstacles(&s, pos, range * range, obl, MAXC) : 0;
+    if (nob > MAXC - 64) nob = MAXC - 64;
+    con cs[MAXC];
+    int start[MAXC + 1];
+    int nObst = 0;
+    const int nc = gen_constraints_ex(prm, st, &s, agent, obl, nob, nbrs, n_nbrs, cs, MAXC, &nObst, start);
+    /* 4 nearest kept edges by (dd, edge) */
+    float nd[4];
+    int ne[4], nci[4];
+    for (int i = 0; i < 4; ++i) { nd[i] = INFINITY; ne[i] = 0x7fffffff; nci[i] = -1; }
+    for (int i = 0; i < nob; ++i) {
+        const int e = obl[i];
+        const v2 c = closest_on_segment(pos, s.p[e], s.p[s.next[e]]);
+        const float dd = sqdist(c, pos);
+        const int ci = start[i + 1] > start[i] ? start[i] : -1;
+        int at = 4;
+        for (int j = 3; j >= 0; --j)
+            if (dd < nd[j] || (dd == nd[j] && e < ne[j])) at = j;
+        if (at < 4) {
+            for (int j = 3; j > at; --j) { nd[j] = nd[j - 1]; ne[j] = ne[j - 1]; nci[j] = nci[j - 1]; }
+            nd[at] = dd; ne[at] = e; nci[at] = ci;
+        }
+    }
+#define FCLAMP(x) fminf(fmaxf((x), -8.0f), 8.0f)
+    for (int i = 0; i < 64; ++i) feat[i] = 0.0f;
+    for (int j = 0; j < n_nbrs && j < 10; ++j) {
+        const int nb = nbrs[j];
+        feat[4 * j + 0] = FCLAMP((st->pos[2 * nb] - pos.x) * 0.25f);
+        feat[4 * j + 1] = FCLAMP((st->pos[2 * nb + 1] - pos.y) * 0.25f);
+        feat[4 * j + 2] = FCLAMP((st->vel[2 * nb] - vel.x) * 0.5f);
+        feat[4 * j + 3] = FCLAMP((st->vel[2 * nb + 1] - vel.y) * 0.5f);
+    }
+    for (int i = 0; i < 4; ++i) {
+        if (ne[i] == 0x7fffffff) continue;
+        const v2 a = s.p[ne[i]], b = s.p[s.next[ne[i]]];
+        feat[40 + 4 * i + 0] = FCLAMP((a.x - pos.x) * 0.25f);
+        feat[40 + 4 * i + 1] = FCLAMP((a.y - pos.y) * 0.25f);
+        feat[40 + 4 * i + 2] = FCLAMP((b.x - pos.x) * 0.25f);
+        feat[40 + 4 * i + 3] = FCLAMP((b.y - pos.y) * 0.25f);
+    }
+    feat[56] = FCLAMP(pref_vel[0] * 0.5f);
+    feat[57] = FCLAMP(pref_vel[1] * 0.5f);
+    feat[58] = FCLAMP(vel.x * 0.5f);
+    feat[59] = FCLAMP(vel.y * 0.5f);
+    feat[60] = FCLAMP(st->radius[agent] * 4.0f);
+#undef FCLAMP
+    const v2 o = V(v[0], v[1]);
+    unsigned act = 0, valid = 0;
+    int repr = 1;
+    for (int c = 0; c < nc && c < MAXC; ++c) {
+        const int a = det(right(cs[c].n), vsub(cs[c].p, o)) >= -1e-3f;
+        int slot = -1;
+        if (c >= nObst) {
+            if (c - nObst < 10) slot = c - nObst;
+        } else {
+            for (int i = 0; i < 4; ++i)
+                if (nci[i] == c) slot = 10 + i;
+        }
+        if (slot >= 0) {
+            valid |= 1u << slot;
+            if (a) act |= 1u << slot;
+        } else if (a) {
+            repr = 0;
+        }
+    }
+    *label = act | (valid << 16) | (repr ? (1u << 30) : 0u);
     obst_free(&s);
     return nc;
 }

@@ -141,6 +141,12 @@ float oracle_cosf(float x);
  * active agents and return its node count. */
 int oracle_kdtree_build_only(const oracle_state* st);
 
+/* Proposal-network inputs (64 floats) and label word of one agent, for
+ * the LP velocity v; see ecm_oracle.c.  Returns the constraint count. */
+int oracle_proposal_sample(const oracle_params* prm, const oracle_state* st, const oracle_obstacles* obs,
+                           int agent, const int* nbrs, int n_nbrs, const float* pref_vel, const float* v,
+                           float* feat, unsigned* label);
+
 #ifdef __cplusplus
 }
 #endif

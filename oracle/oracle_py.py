@@ -81,6 +81,9 @@ def lib():
         for f in ("oracle_atanf", "oracle_sinf", "oracle_cosf"):
             getattr(L, f).restype = C.c_float
             getattr(L, f).argtypes = [C.c_float]
+        L.oracle_proposal_sample.restype = C.c_int
+        L.oracle_proposal_sample.argtypes = [C.POINTER(OParams), C.POINTER(OState), C.POINTER(OObst), C.c_int,
+                                             C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_kdtree_build_only.restype = C.c_int
         L.oracle_kdtree_build_only.argtypes = [C.POINTER(OState)]
         _lib = L
@@ -191,6 +194,20 @@ def lp3d(constraints, n_obst, max_speed, failed, vel):
     v = np.array(vel, np.float32, copy=True)
     lib().oracle_lp3d(_p(c), int(c.shape[0]), int(n_obst), float(max_speed), int(failed), _p(v))
     return v
+
+
+def proposal_sample(params, state, obstacles, agent, nbrs, pref_vel, v):
+    """(features[64] float32 before BF16 rounding, label word) of the
+    proposal network for one agent and LP velocity v."""
+    P = Problem(params, state, obstacles, _EmptyEcm())
+    nb = np.ascontiguousarray(nbrs, np.int32)
+    pv = np.ascontiguousarray(pref_vel, np.float32)
+    vv = np.ascontiguousarray(v, np.float32)
+    feat = np.zeros(64, np.float32)
+    lab = np.zeros(1, np.uint32)
+    lib().oracle_proposal_sample(C.byref(P.prm), C.byref(P.st), C.byref(P.obs), int(agent), _p(nb), int(nb.shape[0]),
+                                 _p(pv), _p(vv), _p(feat), _p(lab))
+    return feat, int(lab[0])
 
 
 def constraints(params, state, obstacles, agent, nbrs):
