@@ -67,6 +67,13 @@ SIGNATURES = {
     "ecm_sim_get_replans": (_I, [_P, _PI, _I, _PI]),
     "ecm_sim_read_paths": (_I, [_P, _PI, _PI, _PF, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ecm_sim_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
+    "ecm_sim_add_env": (_I, [_P, _F, _F, _F, _F, _PI]),
+    "ecm_sim_num_envs": (_I, [_P, _PI]),
+    "ecm_sim_env_set_obstacles": (_I, [_P, _I, _PF, _PI, _I]),
+    "ecm_sim_env_set_ecm": (_I, [_P, _I, _I, _PF, _I, _PI, _PF]),
+    "ecm_sim_env_spawn": (_I, [_P, _I, _I, _PF, _PF, _PF, _PI, _PI, _PF, _I, _PI]),
+    "ecm_sim_env_set_replan_callback": (_I, [_P, _I, REPLAN_FN, _P]),
+    "ecm_sim_agent_envs": (_I, [_P, _PI]),
     "ecm_sim_set_timing": (_I, [_P, _I]),
     "ecm_sim_last_timing": (_I, [_P, _PF, _I]),
     "ecm_sim_snapshot_save": (_I, [_P, C.c_char_p]),
@@ -88,6 +95,7 @@ SIGNATURES = {
     "ecm_scene_inside_obstacle": (_I, [_P, _F, _F]),
     "ecm_scene_replan_callback": (REPLAN_FN, []),
     "ecm_scenario_generate": (_I, [_I, C.c_uint64, _I, C.POINTER(_P), C.POINTER(_P)]),
+    "ecm_scenario_generate_box": (_I, [_F, _I, _I, _I, _I, C.c_uint64, C.POINTER(_P), C.POINTER(_P)]),
     "ecm_scenario_counts": (_I, [_P, _PI, C.POINTER(C.c_longlong)]),
     "ecm_scenario_export": (_I, [_P, _PF, _PF, _PF, _PF, _PI, _PI, _PF]),
     "ecm_scenario_destroy": (None, [_P]),

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -74,9 +75,48 @@ static GridDesc make_grid(float x0, float y0, float x1, float y1, float cell, in
     return g;
 }
 
+// One environment on the host: its scene, device copies and agent grid.
+struct EnvHost {
+    float x0 = 0, y0 = 0, x1 = 0, y1 = 0;       // world box
+    // obstacles
+    std::vector<float> obs_xy;
+    std::vector<int> obs_ring_off;
+    int n_obst_edges = 0;
+    DBuf<float4> oe0, oe1;
+    DBuf<uint8_t> oflags;
+    GridDesc og{};
+    DBuf<int> obucket_start, obucket_edges;
+    int maxo = 16;
+    float obst_range_built = -1.0f;
+    // ECM
+    std::vector<float> ecm_vert;
+    std::vector<int> ecm_target;
+    std::vector<float> ecm_pts;
+    int n_cells = 0;
+    DBuf<float4> cellA, cellB;
+    GridDesc eg{};
+    DBuf<int> ebucket_start, ebucket_cells;
+    int yhash_mask = -1;
+    DBuf<uint32_t> yhash_key;
+    DBuf<int> yhash_off, yhash_cnt, ycells;
+    bool ecm_set = false;
+    // agents and their grid
+    int num_agents = 0;
+    GridDesc ag{};
+    uint32_t cell_base = 0;
+    int grid_for_n = -1;
+    // replanning (UpdatePath through this environment's planner)
+    ecm_replan_fn replan_fn = nullptr;
+    void* replan_user = nullptr;
+    void release() {
+        oe0.release(); oe1.release(); oflags.release(); obucket_start.release(); obucket_edges.release();
+        cellA.release(); cellB.release(); ebucket_start.release(); ebucket_cells.release();
+        yhash_key.release(); yhash_off.release(); yhash_cnt.release(); ycells.release();
+    }
+};
+
 struct ecm_sim {
     ecm_sim_params prm{};
-    float wx0 = 0, wy0 = 0, wx1 = 0, wy1 = 0;
     hipStream_t stream = nullptr;
     int dev = 0;
 
@@ -90,8 +130,6 @@ struct ecm_sim {
     int cur = 0;
 
     // agent grid
-    GridDesc ag{};
-    int grid_for_n = -1;
     DBuf<uint32_t> key, cell_count, cell_start, bsum;
     DBuf<float4> sorted_pr;
     DBuf<float2> sorted_v;
@@ -108,17 +146,16 @@ struct ecm_sim {
     DBuf<uint32_t> counters;
     uint32_t* h_counters = nullptr;
 
-    // obstacles
-    std::vector<float> obs_xy;
-    std::vector<int> obs_ring_off;
-    int n_obst_edges = 0;
-    DBuf<float4> oe0, oe1;
-    DBuf<uint8_t> oflags;
-    GridDesc og{};
-    DBuf<int> obucket_start, obucket_edges;
-    int maxo = 16;
-    float obst_range_built = -1.0f;
+    // environments (0 is created with the handle)
+    std::vector<std::unique_ptr<EnvHost>> envs;
+    DBuf<EnvDesc> d_envs;
+    bool envs_dirty = true;
+    uint32_t n_keys = 0;
+    int maxo = 16;                 // longest obstacle bucket list of any environment (16 / 32 / 64)
     float max_radius = 0.0f, max_pref_speed = 0.0f;   // over every agent spawned
+    DBuf<uint16_t> agent_env;      // environment of each slot (with more than one environment)
+    std::vector<uint16_t> h_env;
+    bool agent_env_dirty = false;
 
     // 3D-LP queue (capacity max_agents)
     DBuf<int> q3d_agent;
@@ -127,19 +164,6 @@ struct ecm_sim {
     DBuf<float2> q3d_vel;
     DBuf<float4> q3d_con;
     int q3d_stride = 0;
-
-    // ECM
-    std::vector<float> ecm_vert;
-    std::vector<int> ecm_target;
-    std::vector<float> ecm_pts;
-    int n_cells = 0;
-    DBuf<float4> cellA, cellB;
-    GridDesc eg{};
-    DBuf<int> ebucket_start, ebucket_cells;
-    int yhash_mask = -1;
-    DBuf<uint32_t> yhash_key;
-    DBuf<int> yhash_off, yhash_cnt, ycells;
-    bool ecm_set = false;
 
     // host mirrors of the slot pool
     std::vector<int> free_stack;         // top = back()
@@ -151,8 +175,6 @@ struct ecm_sim {
     int last_idx = -1;
 
     // callbacks, stats, timing
-    ecm_replan_fn replan_fn = nullptr;
-    void* replan_user = nullptr;
     int replan_threads = 1;
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
@@ -203,10 +225,9 @@ static void free_all(ecm_sim* s) {
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
     s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
-    s->oe0.release(); s->oe1.release(); s->oflags.release();
-    s->obucket_start.release(); s->obucket_edges.release();
-    s->cellA.release(); s->cellB.release(); s->ebucket_start.release(); s->ebucket_cells.release();
-    s->yhash_key.release(); s->yhash_off.release(); s->yhash_cnt.release(); s->ycells.release();
+    for (auto& e : s->envs) e->release();
+    s->d_envs.release();
+    s->agent_env.release();
     if (s->h_counters) (void)hipHostFree(s->h_counters);
     s->h_counters = nullptr;
     if (s->h_replan_ids) (void)hipHostFree(s->h_replan_ids);
@@ -246,7 +267,11 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!ecm_device_available()) return fail(ECM_E_NOGPU, "no gfx950 device available");
     ecm_sim* s = new ecm_sim();
     s->prm = *p;
-    s->wx0 = x0; s->wy0 = y0; s->wx1 = x1; s->wy1 = y1;
+    {
+        std::unique_ptr<EnvHost> e(new EnvHost());
+        e->x0 = x0; e->y0 = y0; e->x1 = x1; e->y1 = y1;
+        s->envs.push_back(std::move(e));
+    }
     const size_t N = (size_t)p->max_agents;
     int rc = 0;
     if (hipGetDevice(&s->dev) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -272,6 +297,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->replan_list.alloc(N);
     if (!rc) rc |= s->counters.alloc(ecm_sim::kNCounters);
     if (!rc) rc |= s->replan_pos.alloc(N);
+    if (!rc) rc |= s->agent_env.alloc(N);
     if (!rc && hipHostMalloc((void**)&s->h_counters, ecm_sim::kNCounters * sizeof(uint32_t)) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipHostMalloc((void**)&s->h_replan_ids, ecm_sim::kReplanPeek * sizeof(int)) != hipSuccess)
@@ -307,6 +333,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     s->h_speed.assign(N, 0.0f);
     s->h_goal.assign(N, make_float2(0.0f, 0.0f));
     s->h_path_cnt.assign(N, 0);
+    s->h_env.assign(N, 0);
     *out = s;
     return 0;
 }
@@ -353,23 +380,23 @@ static double seg_box_dist2(double ax, double ay, double bx, double by, double x
 
 // Bucket every obstacle edge into the cells within `range` of it (capsule
 // test), lists ascending by edge id = (obstacle, vertex) scan order.
-static int build_obstacle_buckets(ecm_sim* s, float range) {
+static int build_obstacle_buckets(ecm_sim* s, EnvHost* E, float range) {
     const float margin = 0.05f;
     const double R = (double)range + margin;
     float cell = 2.0f;
     for (int attempt = 0; attempt < 6; ++attempt, cell *= 0.5f) {
-        GridDesc g = make_grid(s->wx0, s->wy0, s->wx1, s->wy1, cell, 4096);
+        GridDesc g = make_grid(E->x0, E->y0, E->x1, E->y1, cell, 4096);
         const int side = g.side;
         std::vector<std::vector<int>> lists((size_t)side * side);
         // edge endpoints from the host copy
-        const std::vector<int>& ro = s->obs_ring_off;
+        const std::vector<int>& ro = E->obs_ring_off;
         int nr = (int)ro.size() - 1;
         for (int r = 0; r < nr; ++r) {
             int b = ro[r], m = ro[r + 1] - b;
             for (int i = 0; i < m; ++i) {
                 int e = b + i, f = b + (i + 1) % m;
-                double ax = s->obs_xy[2 * e], ay = s->obs_xy[2 * e + 1];
-                double bx = s->obs_xy[2 * f], by = s->obs_xy[2 * f + 1];
+                double ax = E->obs_xy[2 * e], ay = E->obs_xy[2 * e + 1];
+                double bx = E->obs_xy[2 * f], by = E->obs_xy[2 * f + 1];
                 double lx = std::min(ax, bx) - R, hx = std::max(ax, bx) + R;
                 double ly = std::min(ay, by) - R, hy = std::max(ay, by) + R;
                 int cx0 = grid_cell((float)lx, g.x0, g.inv_cell, side), cx1 = grid_cell((float)hx, g.x0, g.inv_cell, side);
@@ -399,23 +426,26 @@ static int build_obstacle_buckets(ecm_sim* s, float range) {
             edges.insert(edges.end(), lists[c].begin(), lists[c].end());
         }
         start[lists.size()] = (int)edges.size();
-        int rc = s->obucket_start.alloc(start.size());
-        if (!rc) rc = s->obucket_edges.alloc(std::max<size_t>(edges.size(), 1));
+        int rc = E->obucket_start.alloc(start.size());
+        if (!rc) rc = E->obucket_edges.alloc(std::max<size_t>(edges.size(), 1));
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(s->obucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(E->obucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
         if (!edges.empty())
-            HIPCHK(hipMemcpyAsync(s->obucket_edges.p, edges.data(), edges.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+            HIPCHK(hipMemcpyAsync(E->obucket_edges.p, edges.data(), edges.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
-        s->og = g;
-        s->maxo = maxlen <= 16 ? 16 : (maxlen <= 32 ? 32 : 64);
-        s->obst_range_built = range;
+        E->og = g;
+        s->envs_dirty = true;
+        E->maxo = maxlen <= 16 ? 16 : (maxlen <= 32 ? 32 : 64);
+        E->obst_range_built = range;
         return 0;
     }
     return fail(ECM_E_STATE, "obstacle bucketing failed");
 }
 
-extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* ring_off, int n_rings) {
+extern "C" int ecm_sim_env_set_obstacles(ecm_sim* s, int env, const float* xy, const int* ring_off, int n_rings) {
     if (!s || n_rings < 0 || (n_rings > 0 && (!xy || !ring_off))) return fail(ECM_E_ARG, "bad obstacle arguments");
+    if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
+    EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
     std::vector<int> ro(ring_off, ring_off + n_rings + 1);
     if (n_rings == 0) ro.assign(1, 0);
@@ -424,13 +454,13 @@ extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* rin
         if (ro[r + 1] - ro[r] < 2) return fail(ECM_E_ARG, "obstacle %d has fewer than 2 vertices (Environment.cpp:288)", r);
     const int nv = ro[n_rings];
     // unchanged rings (a snapshot of the same scene): keep the buckets
-    if (s->obs_ring_off == ro && s->obs_xy.size() == 2 * (size_t)nv &&
-        (nv == 0 || std::memcmp(s->obs_xy.data(), xy, 2 * (size_t)nv * sizeof(float)) == 0))
+    if (E->obs_ring_off == ro && E->obs_xy.size() == 2 * (size_t)nv &&
+        (nv == 0 || std::memcmp(E->obs_xy.data(), xy, 2 * (size_t)nv * sizeof(float)) == 0))
         return 0;
-    s->obs_xy.assign(xy, xy + 2 * (size_t)nv);
-    s->obs_ring_off = ro;
-    s->n_obst_edges = nv;
-    if (nv == 0) { s->obst_range_built = -1.0f; return 0; }
+    E->obs_xy.assign(xy, xy + 2 * (size_t)nv);
+    E->obs_ring_off = ro;
+    E->n_obst_edges = nv;
+    if (nv == 0) { E->obst_range_built = -1.0f; s->envs_dirty = true; return 0; }
     // Obstacle::Initialize (ECMDataTypes.cpp:23-61): prev/next in the ring,
     // isConvex = det(prev - next, p - prev) >= 0 (true for 2-vertex rings).
     std::vector<float4> e0(nv), e1(nv);
@@ -459,16 +489,21 @@ extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* rin
         e1[v] = make_float4(xy[2 * prv[v]], xy[2 * prv[v] + 1], xy[2 * nxt[R]], xy[2 * nxt[R] + 1]);
         fl[v] = (uint8_t)(convex[v] | (convex[R] << 1));
     }
-    int rc = s->oe0.alloc(nv);
-    if (!rc) rc = s->oe1.alloc(nv);
-    if (!rc) rc = s->oflags.alloc(nv);
+    int rc = E->oe0.alloc(nv);
+    if (!rc) rc = E->oe1.alloc(nv);
+    if (!rc) rc = E->oflags.alloc(nv);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(s->oe0.p, e0.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->oe1.p, e1.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->oflags.p, fl.data(), nv, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->oe0.p, e0.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->oe1.p, e1.data(), nv * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->oflags.p, fl.data(), nv, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->obst_range_built = -1.0f;   // (re)bucketed at the next step
+    E->obst_range_built = -1.0f;   // (re)bucketed at the next step
+    s->envs_dirty = true;
     return 0;
+}
+
+extern "C" int ecm_sim_set_obstacles(ecm_sim* s, const float* xy, const int* ring_off, int n_rings) {
+    return ecm_sim_env_set_obstacles(s, 0, xy, ring_off, n_rings);
 }
 
 // ------------------------------------------------------------------- ECM
@@ -477,20 +512,23 @@ static uint32_t hash_u32_host(uint32_t x) {
     return x;
 }
 
-extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int n_edge, const int* he_target,
-                               const float* he_pts) {
+extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float* vert_xy, int n_edge,
+                                   const int* he_target, const float* he_pts) {
     if (!s || n_vert < 0 || n_edge < 0 || (n_edge > 0 && (!vert_xy || !he_target || !he_pts)))
         return fail(ECM_E_ARG, "bad ECM arguments");
+    if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
+    EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
     for (int e = 0; e < 2 * n_edge; ++e)
         if (he_target[e] < 0 || he_target[e] >= n_vert) return fail(ECM_E_ARG, "half-edge target out of range");
-    s->ecm_vert.assign(vert_xy, vert_xy + 2 * (size_t)n_vert);
-    s->ecm_target.assign(he_target, he_target + 2 * (size_t)n_edge);
-    s->ecm_pts.assign(he_pts, he_pts + 8 * (size_t)n_edge);
+    E->ecm_vert.assign(vert_xy, vert_xy + 2 * (size_t)n_vert);
+    E->ecm_target.assign(he_target, he_target + 2 * (size_t)n_edge);
+    E->ecm_pts.assign(he_pts, he_pts + 8 * (size_t)n_edge);
     const int nc = 2 * n_edge;
-    s->n_cells = nc;
-    s->ecm_set = true;
-    if (nc == 0) { s->yhash_mask = -1; return 0; }
+    E->n_cells = nc;
+    E->ecm_set = true;
+    s->envs_dirty = true;
+    if (nc == 0) { E->yhash_mask = -1; return 0; }
     // cells: left (he0.cl, he1.cr), right (he0.cr, he1.cl); ring
     // (v(he1.target), b.p0, b.p1, v(he0.target)) (ECMCellCollection.cpp:18-46, 57-90)
     std::vector<float4> A(nc), B(nc);
@@ -505,7 +543,7 @@ extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int
     }
     // bucket grid: mean cell box area decides the bucket edge
     double area = 0.0;
-    float bx0 = s->wx0, by0 = s->wy0, bx1 = s->wx1, by1 = s->wy1;
+    float bx0 = E->x0, by0 = E->y0, bx1 = E->x1, by1 = E->y1;
     std::vector<float> box(4 * (size_t)nc);
     const float margin = 0.01f;
     for (int c = 0; c < nc; ++c) {
@@ -577,29 +615,34 @@ extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int
         while (tkey[h] != 0xffffffffu) h = (h + 1) & (uint32_t)(tsize - 1);
         tkey[h] = keys[i]; toff[h] = koff[i]; tcnt[h] = kcnt[i];
     }
-    int rc = s->cellA.alloc(nc);
-    if (!rc) rc = s->cellB.alloc(nc);
-    if (!rc) rc = s->ebucket_start.alloc(start.size());
-    if (!rc) rc = s->ebucket_cells.alloc(cells.size());
-    if (!rc) rc = s->yhash_key.alloc(tsize);
-    if (!rc) rc = s->yhash_off.alloc(tsize);
-    if (!rc) rc = s->yhash_cnt.alloc(tsize);
-    if (!rc) rc = s->ycells.alloc(ycells.size());
+    int rc = E->cellA.alloc(nc);
+    if (!rc) rc = E->cellB.alloc(nc);
+    if (!rc) rc = E->ebucket_start.alloc(start.size());
+    if (!rc) rc = E->ebucket_cells.alloc(cells.size());
+    if (!rc) rc = E->yhash_key.alloc(tsize);
+    if (!rc) rc = E->yhash_off.alloc(tsize);
+    if (!rc) rc = E->yhash_cnt.alloc(tsize);
+    if (!rc) rc = E->ycells.alloc(ycells.size());
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(s->cellA.p, A.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->cellB.p, B.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->ebucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->cellA.p, A.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->cellB.p, B.data(), nc * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->ebucket_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
     if (!cells.empty())
-        HIPCHK(hipMemcpyAsync(s->ebucket_cells.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->yhash_key.p, tkey.data(), tsize * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->yhash_off.p, toff.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->yhash_cnt.p, tcnt.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(E->ebucket_cells.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->yhash_key.p, tkey.data(), tsize * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->yhash_off.p, toff.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(E->yhash_cnt.p, tcnt.data(), tsize * sizeof(int), hipMemcpyHostToDevice, s->stream));
     if (!ycells.empty())
-        HIPCHK(hipMemcpyAsync(s->ycells.p, ycells.data(), ycells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(E->ycells.p, ycells.data(), ycells.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->eg = g;
-    s->yhash_mask = (int)tsize - 1;
+    E->eg = g;
+    E->yhash_mask = (int)tsize - 1;
     return 0;
+}
+
+extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int n_edge, const int* he_target,
+                               const float* he_pts) {
+    return ecm_sim_env_set_ecm(s, 0, n_vert, vert_xy, n_edge, he_target, he_pts);
 }
 
 // ------------------------------------------------------------- agents
@@ -631,31 +674,33 @@ struct HostGrid {
     void add(int id, float x, float y) { cells[(size_t)cy(y) * side + cx(x)].push_back(id); }
 };
 
-extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const float* radius, const float* pref_speed,
-                             const int* path_off, const int* path_cnt, const float* path_xy, int validate,
-                             int* out_ids) {
+extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_xy, const float* radius,
+                                 const float* pref_speed, const int* path_off, const int* path_cnt, const float* path_xy,
+                                 int validate, int* out_ids) {
     if (!s || n < 0 || (n > 0 && (!start_xy || !radius || !pref_speed || !path_off || !path_cnt || !path_xy || !out_ids)))
         return fail(ECM_E_ARG, "bad spawn arguments");
+    if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
+    EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
     if (n == 0) return 0;
     for (int i = 0; i < n; ++i)
         if (path_cnt[i] < 1 || path_off[i] < 0)
             return fail(ECM_E_ARG, "agent %d: empty path (the reference reads path.x[-1], Simulator.cpp:554)", i);
-    // ValidSpawnLocation (Simulator.cpp:295-311) against active agents,
-    // including those spawned earlier in this batch.
+    // ValidSpawnLocation (Simulator.cpp:295-311) against the active agents
+    // of the environment, including those spawned earlier in this batch.
     HostGrid hg;
     float rmax_new = 0.0f;
     for (int i = 0; i < n; ++i) rmax_new = std::max(rmax_new, radius[i]);
     const float gcell = std::max(1.0f, rmax_new);
     std::vector<float2> cur_pos;
     if (validate) {
-        hg.init(s->wx0 - 1, s->wy0 - 1, s->wx1 + 1, s->wy1 + 1, gcell);
+        hg.init(E->x0 - 1, E->y0 - 1, E->x1 + 1, E->y1 + 1, gcell);
         if (s->num_agents > 0) {
             cur_pos.resize(s->prm.max_agents);
             HIPCHK(hipMemcpyAsync(cur_pos.data(), s->pos[s->cur].p, cur_pos.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
             for (int i = 0; i <= s->last_idx; ++i)
-                if (s->h_active[i]) hg.add(i, cur_pos[i].x, cur_pos[i].y);
+                if (s->h_active[i] && s->h_env[i] == env) hg.add(i, cur_pos[i].x, cur_pos[i].y);
         }
     }
     std::vector<float2> placed(n);
@@ -689,6 +734,11 @@ extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const flo
         int id = s->free_stack.back();
         s->free_stack.pop_back();
         ++s->num_agents;
+        ++E->num_agents;
+        if (s->h_env[id] != (uint16_t)env) {
+            s->h_env[id] = (uint16_t)env;
+            s->agent_env_dirty = true;
+        }
         s->last_idx = std::max(s->last_idx, id);
         ids[i] = id;
         out_ids[i] = id;
@@ -736,9 +786,16 @@ extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const flo
     return rc;
 }
 
+extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const float* radius, const float* pref_speed,
+                             const int* path_off, const int* path_cnt, const float* path_xy, int validate,
+                             int* out_ids) {
+    return ecm_sim_env_spawn(s, 0, n, start_xy, radius, pref_speed, path_off, path_cnt, path_xy, validate, out_ids);
+}
+
 static void mark_destroyed(ecm_sim* s, int id) {
     // DestroyAgent (Simulator.cpp:202-208)
     --s->num_agents;
+    --s->envs[s->h_env[id]]->num_agents;
     s->h_active[id] = 0;
     s->free_stack.push_back(id);
 }
@@ -780,10 +837,41 @@ extern "C" int ecm_sim_set_path(ecm_sim* s, int id, const float* xy, int n) {
     return 0;
 }
 
-extern "C" int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user) {
+extern "C" int ecm_sim_env_set_replan_callback(ecm_sim* s, int env, ecm_replan_fn fn, void* user) {
     if (!s) return fail(ECM_E_ARG, "null handle");
-    s->replan_fn = fn;
-    s->replan_user = user;
+    if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
+    s->envs[env]->replan_fn = fn;
+    s->envs[env]->replan_user = user;
+    return 0;
+}
+
+extern "C" int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user) {
+    return ecm_sim_env_set_replan_callback(s, 0, fn, user);
+}
+
+// ---------------------------------------------------------- environments
+extern "C" int ecm_sim_add_env(ecm_sim* s, float x0, float y0, float x1, float y1, int* env) {
+    if (!s || !env) return fail(ECM_E_ARG, "null argument");
+    if (!(x1 > x0) || !(y1 > y0)) return fail(ECM_E_ARG, "empty world box");
+    if (s->envs.size() >= 65535) return fail(ECM_E_FULL, "too many environments");
+    std::unique_ptr<EnvHost> e(new EnvHost());
+    e->x0 = x0; e->y0 = y0; e->x1 = x1; e->y1 = y1;
+    s->envs.push_back(std::move(e));
+    s->envs_dirty = true;
+    for (auto& q : s->envs) q->grid_for_n = -1;       // new key layout at the next step
+    *env = (int)s->envs.size() - 1;
+    return 0;
+}
+
+extern "C" int ecm_sim_num_envs(ecm_sim* s, int* n) {
+    if (!s || !n) return fail(ECM_E_ARG, "null argument");
+    *n = (int)s->envs.size();
+    return 0;
+}
+
+extern "C" int ecm_sim_agent_envs(ecm_sim* s, int* env) {
+    if (!s || !env) return fail(ECM_E_ARG, "null argument");
+    for (int i = 0; i < s->prm.max_agents; ++i) env[i] = s->h_active[i] ? (int)s->h_env[i] : -1;
     return 0;
 }
 
@@ -794,20 +882,34 @@ extern "C" int ecm_sim_set_replan_threads(ecm_sim* s, int n) {
 }
 
 // ------------------------------------------------------------------ step
+// Agent grid of every environment, sized by its agent count (about one
+// agent per cell unless grid_cell fixes the edge), laid out one after the
+// other in a shared key space.  Rebuilt when an environment's count leaves
+// [n / 2, 2 n] of the count its grid was made for.
 static int ensure_agent_grid(ecm_sim* s) {
-    int n = std::max(1, s->num_agents);
-    if (s->grid_for_n > 0 && n <= 2 * s->grid_for_n && 2 * n >= s->grid_for_n && s->prm.grid_cell <= 0.0f) return 0;
-    if (s->grid_for_n > 0 && s->prm.grid_cell > 0.0f) return 0;
-    float ext = std::max(s->wx1 - s->wx0, s->wy1 - s->wy0);
-    float cell = s->prm.grid_cell;
-    if (cell <= 0.0f) {
-        double area = (double)(s->wx1 - s->wx0) * (s->wy1 - s->wy0);
-        cell = (float)std::sqrt(area / n) * 1.0f;
-        cell = std::min(std::max(cell, 0.5f), 32.0f);
+    bool rebuild = false;
+    for (auto& e : s->envs) {
+        const int n = std::max(1, e->num_agents);
+        if (e->grid_for_n < 0) rebuild = true;
+        else if (s->prm.grid_cell <= 0.0f && (n > 2 * e->grid_for_n || 2 * n < e->grid_for_n)) rebuild = true;
     }
-    GridDesc g = make_grid(s->wx0, s->wy0, s->wx1, s->wy1, cell, 2048);
-    (void)ext;
-    const size_t G = (size_t)1 << (2 * g.bits);
+    if (!rebuild) return 0;
+    uint64_t base = 0;
+    for (auto& e : s->envs) {
+        const int n = std::max(1, e->num_agents);
+        float cell = s->prm.grid_cell;
+        if (cell <= 0.0f) {
+            double area = (double)(e->x1 - e->x0) * (e->y1 - e->y0);
+            cell = (float)std::sqrt(area / n);
+            cell = std::min(std::max(cell, 0.5f), 32.0f);
+        }
+        e->ag = make_grid(e->x0, e->y0, e->x1, e->y1, cell, 2048);
+        e->cell_base = (uint32_t)base;
+        e->grid_for_n = n;
+        base += (uint64_t)1 << (2 * e->ag.bits);
+    }
+    if (base >= 0xffffffffull) return fail(ECM_E_STATE, "agent grids of the environments exceed 2^32 cells");
+    const size_t G = (size_t)base;
     int rc = s->cell_count.alloc(G);
     if (!rc) rc = s->cell_start.alloc(G + 1);
     if (!rc) rc = s->bsum.alloc((G + 1023) / 1024 + 1);
@@ -815,8 +917,42 @@ static int ensure_agent_grid(ecm_sim* s) {
     rc = zero_dev(s->cell_count.p, G * sizeof(uint32_t), s->stream);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->ag = g;
-    s->grid_for_n = n;
+    s->n_keys = (uint32_t)G;
+    s->envs_dirty = true;
+    return 0;
+}
+
+// Device copy of the environment descriptors and of the slot -> environment
+// map, refreshed when the host side changed.
+static int upload_envs(ecm_sim* s) {
+    if (s->envs_dirty) {
+        std::vector<EnvDesc> h(s->envs.size());
+        for (size_t i = 0; i < s->envs.size(); ++i) {
+            const EnvHost& E = *s->envs[i];
+            EnvDesc& d = h[i];
+            std::memset(&d, 0, sizeof(d));
+            d.ag = E.ag; d.cell_base = E.cell_base;
+            d.n_obst_edges = E.n_obst_edges; d.oe0 = E.oe0.p; d.oe1 = E.oe1.p; d.oflags = E.oflags.p; d.og = E.og;
+            d.obucket_start = E.obucket_start.p; d.obucket_edges = E.obucket_edges.p;
+            d.n_cells = E.n_cells; d.cellA = E.cellA.p; d.cellB = E.cellB.p; d.eg = E.eg;
+            d.ebucket_start = E.ebucket_start.p; d.ebucket_cells = E.ebucket_cells.p;
+            d.yhash_mask = E.yhash_mask; d.yhash_key = E.yhash_key.p; d.yhash_off = E.yhash_off.p;
+            d.yhash_cnt = E.yhash_cnt.p; d.ycells = E.ycells.p;
+        }
+        if (s->d_envs.n < h.size()) {
+            int rc = s->d_envs.alloc(h.size());
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(s->d_envs.p, h.data(), h.size() * sizeof(EnvDesc), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->envs_dirty = false;
+    }
+    if (s->agent_env_dirty) {
+        HIPCHK(hipMemcpyAsync(s->agent_env.p, s->h_env.data(), s->h_env.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                              s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->agent_env_dirty = false;
+    }
     return 0;
 }
 
@@ -830,7 +966,10 @@ static DevState make_state(ecm_sim* s) {
     d.attraction = s->attraction.p; d.pref_vel = s->pref_vel.p; d.active = s->active.p;
     d.path_off = s->path_off.p; d.path_cnt = s->path_cnt.p; d.path_pts = s->path_pool.p;
     d.v_orca = s->v_orca.p;
-    d.ag = s->ag;
+    d.n_env = (int)s->envs.size();
+    d.envs = s->d_envs.p;
+    d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
+    d.n_keys = s->n_keys;
     d.key = s->key.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
     d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
     d.destroyed_list = s->destroyed_list.p; d.destroyed_n = s->counters.p;
@@ -852,13 +991,6 @@ static DevState make_state(ecm_sim* s) {
     }
     d.q3d_agent = s->q3d_agent.p; d.q3d_info = s->q3d_info.p; d.q3d_vel = s->q3d_vel.p;
     d.q3d_con = s->q3d_con.p; d.con_stride = s->q3d_stride;
-    d.n_obst_edges = s->n_obst_edges;
-    d.oe0 = s->oe0.p; d.oe1 = s->oe1.p; d.oflags = s->oflags.p; d.og = s->og;
-    d.obucket_start = s->obucket_start.p; d.obucket_edges = s->obucket_edges.p;
-    d.n_cells = s->n_cells; d.cellA = s->cellA.p; d.cellB = s->cellB.p; d.eg = s->eg;
-    d.ebucket_start = s->ebucket_start.p; d.ebucket_cells = s->ebucket_cells.p;
-    d.yhash_mask = s->yhash_mask; d.yhash_key = s->yhash_key.p; d.yhash_off = s->yhash_off.p;
-    d.yhash_cnt = s->yhash_cnt.p; d.ycells = s->ycells.p;
     d.k = s->prm.k;
     d.time_horizon = s->prm.time_horizon;
     d.time_horizon_obst = s->prm.time_horizon_obst;
@@ -886,7 +1018,10 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
         res[i].resize(2 * cap);
         float2 g = s->h_goal[id];
         int np = 0;
-        int r = s->replan_fn(s->replan_user, id, rpos[i].x, rpos[i].y, g.x, g.y, s->h_radius[id], res[i].data(), cap, &np);
+        const EnvHost* E = s->envs[s->h_env[id]].get();
+        int r = E->replan_fn ? E->replan_fn(E->replan_user, id, rpos[i].x, rpos[i].y, g.x, g.y, s->h_radius[id],
+                                            res[i].data(), cap, &np)
+                             : 1;
         cnt[i] = (r == 0 && np >= 1) ? np : 0;     // an empty result keeps the old path
     };
     int nt = std::min(std::max(1, s->replan_threads), n);
@@ -935,13 +1070,20 @@ static int step_once(ecm_sim* s) {
     while (s->last_idx >= 0 && !s->h_active[s->last_idx]) --s->last_idx;
     int rc = ensure_agent_grid(s);
     if (rc) return rc;
-    if (s->n_obst_edges > 0) {
-        float range = obstacle_range(s);
-        if (s->obst_range_built < range) {
-            rc = build_obstacle_buckets(s, range * 1.25f + 0.5f);
-            if (rc) return rc;
+    {
+        const float range = obstacle_range(s);
+        int maxo = 16;
+        for (auto& e : s->envs) {
+            if (e->n_obst_edges > 0 && e->obst_range_built < range) {
+                rc = build_obstacle_buckets(s, e.get(), range * 1.25f + 0.5f);
+                if (rc) return rc;
+            }
+            if (e->n_obst_edges > 0) maxo = std::max(maxo, e->maxo);
         }
+        s->maxo = maxo;
     }
+    rc = upload_envs(s);
+    if (rc) return rc;
     {
         int stride = s->maxo + s->prm.k;
         if (stride != s->q3d_stride) {
@@ -975,7 +1117,7 @@ static int step_once(ecm_sim* s) {
         // learned proposal: features, network, then the verifying ORCA pass
         mode = 2;
         HIPCHK(launch_features(d, s->stream));
-        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + ((size_t)1 << (2 * s->ag.bits)), 0, s->mlp->frags.p,
+        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + s->n_keys, 0, s->mlp->frags.p,
                                      s->mlp->pbias.p, s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu,
                                      s->stream));
     }
@@ -1005,7 +1147,9 @@ static int step_once(ecm_sim* s) {
         rpos.swap(p2);
     }
     s->h_replans = rid;
-    if (nr && s->replan_fn) {
+    bool any_planner = false;
+    for (auto& e : s->envs) any_planner |= e->replan_fn != nullptr;
+    if (nr && any_planner) {
         rc = run_replans(s, rid, rpos);
         if (rc) return rc;
     }
@@ -1045,7 +1189,8 @@ static int step_once(ecm_sim* s) {
 
 extern "C" int ecm_sim_step(ecm_sim* s, int n_steps) {
     if (!s || n_steps < 0) return fail(ECM_E_ARG, "bad step arguments");
-    if (!s->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before step");
+    for (size_t e = 0; e < s->envs.size(); ++e)
+        if (!s->envs[e]->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before step (environment %zu)", e);
     (void)hipSetDevice(s->dev);
     for (int i = 0; i < n_steps; ++i) {
         int rc = step_once(s);
@@ -1145,7 +1290,7 @@ extern "C" int ecm_sim_last_index(ecm_sim* s, int* out) {
 
 extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
     if (!s || !nbr || !cnt) return fail(ECM_E_ARG, "null argument");
-    if (s->grid_for_n < 0) return fail(ECM_E_STATE, "no step has run");
+    if (s->n_keys == 0) return fail(ECM_E_STATE, "no step has run");
     (void)hipSetDevice(s->dev);
     const size_t N = (size_t)s->prm.max_agents;
     DBuf<int> dn, dc;
@@ -1258,6 +1403,9 @@ static bool wr(FILE* f, const T* p, size_t n) { return fwrite(p, sizeof(T), n, f
 template <typename T>
 static bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
 
+// Snapshot layout (version 2): header, parameters, counts, the agent SoA,
+// the path pool, the free-slot stack, the slot -> environment map, then per
+// environment its world box, obstacles, ECM and agent count.
 extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
     if (!s || !path) return fail(ECM_E_ARG, "null argument");
     const size_t N = (size_t)s->prm.max_agents;
@@ -1279,18 +1427,23 @@ extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
     if (s->path_used) HIPCHK(hipMemcpy(pool.data(), s->path_pool.p, s->path_used * sizeof(float2), hipMemcpyDeviceToHost));
     FILE* f = fopen(path, "wb");
     if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
-    uint32_t hdr[4] = {kSnapMagic, 1u, (uint32_t)N, (uint32_t)sizeof(ecm_sim_params)};
-    float box[4] = {s->wx0, s->wy0, s->wx1, s->wy1};
+    uint32_t hdr[4] = {kSnapMagic, 2u, (uint32_t)N, (uint32_t)sizeof(ecm_sim_params)};
     uint64_t pu = s->path_used, fs = s->free_stack.size();
-    int32_t ints[4] = {s->num_agents, s->last_idx, (int32_t)s->obs_ring_off.size(), (int32_t)(s->ecm_vert.size() / 2)};
-    int32_t ne = (int32_t)(s->ecm_target.size() / 2);
-    bool ok = wr(f, hdr, 4) && wr(f, &s->prm, 1) && wr(f, box, 4) && wr(f, ints, 4) && wr(f, &ne, 1) && wr(f, &pu, 1) &&
-              wr(f, &fs, 1) && wr(f, pos.data(), N) && wr(f, vel.data(), N) && wr(f, attr.data(), N) &&
-              wr(f, pv.data(), N) && wr(f, rad.data(), N) && wr(f, spd.data(), N) && wr(f, act.data(), N) &&
-              wr(f, off.data(), N) && wr(f, cnt.data(), N) && wr(f, pool.data(), pool.size()) &&
-              wr(f, s->free_stack.data(), fs) && wr(f, s->obs_ring_off.data(), s->obs_ring_off.size()) &&
-              wr(f, s->obs_xy.data(), s->obs_xy.size()) && wr(f, s->ecm_vert.data(), s->ecm_vert.size()) &&
-              wr(f, s->ecm_target.data(), s->ecm_target.size()) && wr(f, s->ecm_pts.data(), s->ecm_pts.size());
+    int32_t ints[3] = {s->num_agents, s->last_idx, (int32_t)s->envs.size()};
+    bool ok = wr(f, hdr, 4) && wr(f, &s->prm, 1) && wr(f, ints, 3) && wr(f, &pu, 1) && wr(f, &fs, 1) &&
+              wr(f, pos.data(), N) && wr(f, vel.data(), N) && wr(f, attr.data(), N) && wr(f, pv.data(), N) &&
+              wr(f, rad.data(), N) && wr(f, spd.data(), N) && wr(f, act.data(), N) && wr(f, off.data(), N) &&
+              wr(f, cnt.data(), N) && wr(f, pool.data(), pool.size()) && wr(f, s->free_stack.data(), fs) &&
+              wr(f, s->h_env.data(), N);
+    for (size_t e = 0; ok && e < s->envs.size(); ++e) {
+        const EnvHost& E = *s->envs[e];
+        float box[4] = {E.x0, E.y0, E.x1, E.y1};
+        int32_t c[4] = {(int32_t)E.obs_ring_off.size(), (int32_t)(E.ecm_vert.size() / 2),
+                        (int32_t)(E.ecm_target.size() / 2), E.num_agents};
+        ok = wr(f, box, 4) && wr(f, c, 4) && wr(f, E.obs_ring_off.data(), E.obs_ring_off.size()) &&
+             wr(f, E.obs_xy.data(), E.obs_xy.size()) && wr(f, E.ecm_vert.data(), E.ecm_vert.size()) &&
+             wr(f, E.ecm_target.data(), E.ecm_target.size()) && wr(f, E.ecm_pts.data(), E.ecm_pts.size());
+    }
     fclose(f);
     if (!ok) return fail(ECM_E_ARG, "write to %s failed", path);
     return 0;
@@ -1302,40 +1455,67 @@ extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
     if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
     uint32_t hdr[4];
     ecm_sim_params prm;
-    float box[4];
-    int32_t ints[4], ne;
+    int32_t ints[3];
     uint64_t pu, fs;
-    bool ok = rd(f, hdr, 4) && hdr[0] == kSnapMagic && hdr[1] == 1u && hdr[3] == sizeof(ecm_sim_params) &&
-              rd(f, &prm, 1) && rd(f, box, 4) && rd(f, ints, 4) && rd(f, &ne, 1) && rd(f, &pu, 1) && rd(f, &fs, 1);
-    if (!ok || (int)hdr[2] != s->prm.max_agents || prm.k != s->prm.k) {
+    bool ok = rd(f, hdr, 4) && hdr[0] == kSnapMagic && hdr[1] == 2u && hdr[3] == sizeof(ecm_sim_params) &&
+              rd(f, &prm, 1) && rd(f, ints, 3) && rd(f, &pu, 1) && rd(f, &fs, 1);
+    if (!ok || (int)hdr[2] != s->prm.max_agents || prm.k != s->prm.k || ints[2] < 1 ||
+        ints[2] < (int)s->envs.size()) {
         fclose(f);
-        return fail(ECM_E_ARG, "%s: not a snapshot for this handle (max_agents / k differ)", path);
+        return fail(ECM_E_ARG, "%s: not a snapshot for this handle (max_agents / k / environments differ)", path);
     }
     const size_t N = hdr[2];
     std::vector<float2> pos(N), vel(N), attr(N), pv(N), pool(pu);
     std::vector<float> rad(N), spd(N);
     std::vector<uint8_t> act(N);
-    std::vector<int> off(N), cnt(N), stack(fs), ro(ints[2]);
-    std::vector<float> oxy, vxy, pts;
-    std::vector<int> tgt;
+    std::vector<int> off(N), cnt(N), stack(fs);
+    std::vector<uint16_t> henv(N);
     ok = rd(f, pos.data(), N) && rd(f, vel.data(), N) && rd(f, attr.data(), N) && rd(f, pv.data(), N) &&
          rd(f, rad.data(), N) && rd(f, spd.data(), N) && rd(f, act.data(), N) && rd(f, off.data(), N) &&
-         rd(f, cnt.data(), N) && rd(f, pool.data(), pu) && rd(f, stack.data(), fs) && rd(f, ro.data(), ro.size());
-    if (ok) {
-        int nv = ro.empty() ? 0 : ro.back();
-        oxy.resize(2 * (size_t)nv);
-        vxy.resize(2 * (size_t)ints[3]);
-        tgt.resize(2 * (size_t)ne);
-        pts.resize(8 * (size_t)ne);
-        ok = rd(f, oxy.data(), oxy.size()) && rd(f, vxy.data(), vxy.size()) && rd(f, tgt.data(), tgt.size()) &&
-             rd(f, pts.data(), pts.size());
+         rd(f, cnt.data(), N) && rd(f, pool.data(), pu) && rd(f, stack.data(), fs) && rd(f, henv.data(), N);
+    struct EnvData {
+        float box[4];
+        int32_t c[4];
+        std::vector<int> ro;
+        std::vector<float> oxy, vxy, pts;
+        std::vector<int> tgt;
+    };
+    std::vector<EnvData> ed(ints[2]);
+    for (int e = 0; ok && e < ints[2]; ++e) {
+        EnvData& d = ed[e];
+        ok = rd(f, d.box, 4) && rd(f, d.c, 4) && d.c[0] >= 1;
+        if (!ok) break;
+        d.ro.resize(d.c[0]);
+        ok = rd(f, d.ro.data(), d.ro.size());
+        if (!ok) break;
+        d.oxy.resize(2 * (size_t)d.ro.back());
+        d.vxy.resize(2 * (size_t)d.c[1]);
+        d.tgt.resize(2 * (size_t)d.c[2]);
+        d.pts.resize(8 * (size_t)d.c[2]);
+        ok = rd(f, d.oxy.data(), d.oxy.size()) && rd(f, d.vxy.data(), d.vxy.size()) && rd(f, d.tgt.data(), d.tgt.size()) &&
+             rd(f, d.pts.data(), d.pts.size());
     }
     fclose(f);
     if (!ok) return fail(ECM_E_ARG, "%s: truncated snapshot", path);
     s->prm = prm;
-    s->wx0 = box[0]; s->wy0 = box[1]; s->wx1 = box[2]; s->wy1 = box[3];
-    int rc = ecm_sim_set_obstacles(s, oxy.data(), ro.data(), (int)ro.size() - 1);
-    if (!rc) rc = ecm_sim_set_ecm(s, ints[3], vxy.data(), ne, tgt.data(), pts.data());
+    int rc = 0;
+    for (int e = 0; !rc && e < ints[2]; ++e) {
+        const EnvData& d = ed[e];
+        if (e >= (int)s->envs.size()) {
+            int id = 0;
+            rc = ecm_sim_add_env(s, d.box[0], d.box[1], d.box[2], d.box[3], &id);
+            if (rc) break;
+        }
+        EnvHost* E = s->envs[e].get();
+        if (E->x0 != d.box[0] || E->y0 != d.box[1] || E->x1 != d.box[2] || E->y1 != d.box[3]) {
+            E->x0 = d.box[0]; E->y0 = d.box[1]; E->x1 = d.box[2]; E->y1 = d.box[3];
+            E->obst_range_built = -1.0f;
+        }
+        rc = ecm_sim_env_set_obstacles(s, e, d.oxy.data(), d.ro.data(), (int)d.ro.size() - 1);
+        if (!rc) rc = ecm_sim_env_set_ecm(s, e, d.c[1], d.vxy.data(), d.c[2], d.tgt.data(), d.pts.data());
+        E->num_agents = d.c[3];
+        E->grid_for_n = -1;
+    }
     if (!rc) rc = ensure_path_capacity(s, pu > s->path_used ? pu - s->path_used : 0);
     if (rc) return rc;
     (void)hipSetDevice(s->dev);
@@ -1354,10 +1534,11 @@ extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
     s->free_stack = stack;
     s->num_agents = ints[0];
     s->last_idx = ints[1];
+    s->h_env = henv;
+    s->agent_env_dirty = true;
     for (size_t i = 0; i < N; ++i) {
         s->h_path_cnt[i] = cnt[i];
         s->h_goal[i] = cnt[i] > 0 ? pool[off[i] + cnt[i] - 1] : make_float2(0.0f, 0.0f);
     }
-    s->grid_for_n = -1;
     return 0;
 }

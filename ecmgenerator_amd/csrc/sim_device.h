@@ -17,6 +17,37 @@ struct GridDesc {
     int bits;           // Morton bits per axis (side <= 1 << bits)
 };
 
+// One environment: an independent scene (obstacles, ECM, world box) whose
+// agents only ever interact with each other.  A simulation holds one or
+// more; batched multi-scenario stepping advances all of them in the same
+// launches.
+struct EnvDesc {
+    // agent grid: cells [cell_base, cell_base + 4^ag.bits) of the shared key space
+    GridDesc ag;
+    uint32_t cell_base;
+    // obstacles (Environment::GetObstacles, edge per vertex in scan order)
+    int n_obst_edges;
+    const float4* oe0;      // (L.x, L.y, R.x, R.y),  R = next(L)
+    const float4* oe1;      // (prev(L).x, prev(L).y, next(R).x, next(R).y)
+    const uint8_t* oflags;  // bit0 convex(L), bit1 convex(R)
+    GridDesc og;
+    const int* obucket_start;   // [side^2 + 1]
+    const int* obucket_edges;   // ascending edge ids
+    // ECM cells (two per edge, ECMCellCollection.cpp:18-46)
+    int n_cells;
+    const float4* cellA;    // (q0, q1) = (v(he1.target), boundary.p0)
+    const float4* cellB;    // (q2, q3) = (boundary.p1, v(he0.target))
+    GridDesc eg;
+    const int* ebucket_start;   // [side^2 + 1]
+    const int* ebucket_cells;   // ascending cell ids
+    // exact-y fallback: cells having a vertex whose y equals a given float
+    int yhash_mask;             // table size - 1 (power of two), -1 if empty
+    const uint32_t* yhash_key;  // y bit pattern (+0.0 for zero), 0xffffffff empty
+    const int* yhash_off;       // [table] start into ycells
+    const int* yhash_cnt;       // [table]
+    const int* ycells;          // ascending cell ids per key
+};
+
 // Everything the step kernels read or write.  Agent arrays are indexed by
 // slot id (entity == slot, Simulator.cpp:40); "sorted" arrays by the
 // position of the agent in the Morton-ordered grid.
@@ -35,11 +66,16 @@ struct DevState {
     const float2* path_pts;
     float2* v_orca;                              // ORCA output velocity (diagnostic, may be null)
 
+    // ---- environments ----
+    int n_env;
+    const EnvDesc* envs;         // [n_env]
+    const uint16_t* agent_env;   // [n_slots] environment of each slot, null: all in environment 0
+
     // ---- agent grid (rebuilt every step) ----
-    GridDesc ag;
-    uint32_t* key;          // [n_slots] Morton key or 0xffffffff
-    uint32_t* cell_count;   // [4^bits]   zero between steps
-    uint32_t* cell_start;   // [4^bits + 1] exclusive scan; [4^bits] = active count
+    uint32_t n_keys;        // cells of all environments
+    uint32_t* key;          // [n_slots] cell key or 0xffffffff
+    uint32_t* cell_count;   // [n_keys]   zero between steps
+    uint32_t* cell_start;   // [n_keys + 1] exclusive scan; [n_keys] = active count
     float4* sorted_pr;      // [n_slots] (x, y, radius, id bits)
     float2* sorted_v;       // [n_slots] (vx, vy)
     int* nbr;               // [k][n_slots] neighbour sorted positions (SoA)
@@ -70,29 +106,6 @@ struct DevState {
     float2* log_v;              // [log_cap] LP velocity
     uint32_t* log_n;            // append counter
     uint32_t log_cap;
-
-    // ---- obstacles (Environment::GetObstacles, edge per vertex in scan order) ----
-    int n_obst_edges;
-    const float4* oe0;      // (L.x, L.y, R.x, R.y),  R = next(L)
-    const float4* oe1;      // (prev(L).x, prev(L).y, next(R).x, next(R).y)
-    const uint8_t* oflags;  // bit0 convex(L), bit1 convex(R)
-    GridDesc og;
-    const int* obucket_start;   // [side^2 + 1]
-    const int* obucket_edges;   // ascending edge ids
-
-    // ---- ECM cells (two per edge, ECMCellCollection.cpp:18-46) ----
-    int n_cells;
-    const float4* cellA;    // (q0, q1) = (v(he1.target), boundary.p0)
-    const float4* cellB;    // (q2, q3) = (boundary.p1, v(he0.target))
-    GridDesc eg;
-    const int* ebucket_start;   // [side^2 + 1]
-    const int* ebucket_cells;   // ascending cell ids
-    // exact-y fallback: cells having a vertex whose y equals a given float
-    int yhash_mask;             // table size - 1 (power of two), -1 if empty
-    const uint32_t* yhash_key;  // y bit pattern (+0.0 for zero), 0xffffffff empty
-    const int* yhash_off;       // [table] start into ycells
-    const int* yhash_cnt;       // [table]
-    const int* ycells;          // ascending cell ids per key
 
     // ---- parameters (ORCA.h:85-103, Simulator.cpp:538-686) ----
     int k;

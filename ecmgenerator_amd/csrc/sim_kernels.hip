@@ -37,7 +37,9 @@ __device__ __forceinline__ uint32_t part1by1(uint32_t x) {
 __device__ __forceinline__ uint32_t morton2(uint32_t cx, uint32_t cy) {
     return part1by1(cx) | (part1by1(cy) << 1);
 }
-__device__ __forceinline__ uint32_t n_grid_keys(const GridDesc& g) { return 1u << (2 * g.bits); }
+__device__ __forceinline__ const EnvDesc& env_of(const DevState& s, int id) {
+    return s.envs[s.agent_env ? s.agent_env[id] : 0];
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i -> XCD i % 8).
 // Remap so that XCD x processes the x-th contiguous eighth of the Morton
@@ -53,9 +55,10 @@ __global__ void k_grid_count(DevState s) {
     if (i >= s.n_slots) return;
     if (!s.active[i]) { s.key[i] = kInvalidKey; return; }
     float2 p = s.pos_in[i];
-    int cx = grid_cell(p.x, s.ag.x0, s.ag.inv_cell, s.ag.side);
-    int cy = grid_cell(p.y, s.ag.y0, s.ag.inv_cell, s.ag.side);
-    uint32_t m = morton2((uint32_t)cx, (uint32_t)cy);
+    const EnvDesc& E = env_of(s, i);
+    int cx = grid_cell(p.x, E.ag.x0, E.ag.inv_cell, E.ag.side);
+    int cy = grid_cell(p.y, E.ag.y0, E.ag.inv_cell, E.ag.side);
+    uint32_t m = E.cell_base + morton2((uint32_t)cx, (uint32_t)cy);
     s.key[i] = m;
     atomicAdd(&s.cell_count[m], 1u);
 }
@@ -147,7 +150,7 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
 // linear scan of ECMCellCollection::PointLocationQueryLinear: a cell can only
 // contain p if p lies in its (margin-expanded) box, except when p.y equals
 // the y of one of its vertices exactly; those cells are listed per y value.
-__device__ int locate_cell(const DevState& s, V2 p, uint32_t& fallback) {
+__device__ int locate_cell(const EnvDesc& s, V2 p, uint32_t& fallback) {
     if (s.n_cells == 0) return -1;
     int bx = grid_cell(p.x, s.eg.x0, s.eg.inv_cell, s.eg.side);
     int by = grid_cell(p.y, s.eg.y0, s.eg.inv_cell, s.eg.side);
@@ -179,7 +182,7 @@ __device__ int locate_cell(const DevState& s, V2 p, uint32_t& fallback) {
 
 // ECM::RetractPoint (ECM.cpp:20-96) given the located cell.  Cell 2e holds
 // (p1, he0.cl, he1.cr, p2), cell 2e+1 holds (p1, he0.cr, he1.cl, p2).
-__device__ bool retract(const DevState& s, V2 loc, int cell, V2& out, V2& oa, V2& ob) {
+__device__ bool retract(const EnvDesc& s, V2 loc, int cell, V2& out, V2& oa, V2& ob) {
     int e2 = cell & ~1;
     float4 A0 = s.cellA[e2], B0 = s.cellB[e2];
     float4 A1 = s.cellA[e2 + 1], B1 = s.cellB[e2 + 1];
@@ -202,7 +205,7 @@ __device__ bool retract(const DevState& s, V2 loc, int cell, V2& out, V2& oa, V2
 }
 
 // IRMPathFollower::FindAttractionPoint (IRMPathFollower.cpp:14-115)
-__device__ bool irm_attraction(const DevState& s, V2 pos, const float2* path, int np, V2& out, uint32_t& fb) {
+__device__ bool irm_attraction(const EnvDesc& s, V2 pos, const float2* path, int np, V2& out, uint32_t& fb) {
     int cell = locate_cell(s, pos, fb);
     if (cell < 0) return false;
     V2 rl = mk(0.0f, 0.0f), oa, ob;
@@ -286,8 +289,8 @@ __device__ __forceinline__ float nb_kth(const float (&d)[K], int k) {
 // whole unvisited region is.  A 2 mm guard covers rounding.  The list is
 // ordered by (d2, id), so the result does not depend on visiting order.
 template <int K, bool EXACT>
-__device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot)[K]) {
-    const GridDesc& g = s.ag;
+__device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2 p, int (&out_slot)[K]) {
+    const GridDesc& g = E.ag;
     const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
     const int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
     float d[K];
@@ -319,7 +322,7 @@ __device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot
                 float ddy = fmaxf(fmaxf(by0 - p.y, p.y - by1), 0.0f);
                 if (ddx * ddx + ddy * ddy > prune2) continue;
             }
-            const uint32_t m = morton2((uint32_t)x, (uint32_t)y);
+            const uint32_t m = E.cell_base + morton2((uint32_t)x, (uint32_t)y);
             const uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
             for (uint32_t j = a; j < b; ++j) {
                 float4 c = s.sorted_pr[j];
@@ -354,12 +357,13 @@ __device__ __forceinline__ int knn_query(const DevState& s, V2 p, int (&out_slot
 // --------------------------------------- attraction + steering + neighbours
 template <int K, bool EXACT>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
-    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
     float4 pr = s.sorted_pr[t];
     const int id = __float_as_int(pr.w);
     const V2 pos = mk(pr.x, pr.y);
+    const EnvDesc& E = env_of(s, id);
 
     // UpdateAttractionPointSystem (Simulator.cpp:538-590)
     const int off = s.path_off[id], np = s.path_cnt[id];
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
         if (dGoal < s.delete_r2) alive = false;
     } else {
         V2 ap = mk(0.0f, 0.0f);
-        if (irm_attraction(s, pos, path, np, ap, fb)) attr = ap;
+        if (irm_attraction(E, pos, path, np, ap, fb)) attr = ap;
         else {
             // UpdatePath(e, currentPosition, goal) with the step-start position
             uint32_t q = atomicAdd(s.replan_n, 1u);
@@ -400,7 +404,7 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
-    const int cnt = knn_query<K, EXACT>(s, pos, slot);
+    const int cnt = knn_query<K, EXACT>(s, E, pos, slot);
 #pragma unroll
     for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
@@ -425,10 +429,10 @@ __device__ __forceinline__ float std_max(float a, float b) { return (a < b) ? b 
 
 // Obstacle half-plane for edge e (ORCA.cpp:70-333).  Returns false when the
 // reference emits nothing for it (`continue` without push_back).
-__device__ bool obstacle_constraint(const DevState& s, int e, V2 position, V2 velocity, float rad,
+__device__ bool obstacle_constraint(const DevState& s, const EnvDesc& E, int e, V2 position, V2 velocity, float rad,
                                     V2& outP, V2& outN) {
-    float4 a = s.oe0[e], b = s.oe1[e];
-    uint8_t fl = s.oflags[e];
+    float4 a = E.oe0[e], b = E.oe1[e];
+    uint8_t fl = E.oflags[e];
     V2 pL = mk(a.x, a.y), pR = mk(a.z, a.w);
     V2 pPrevL = mk(b.x, b.y), pNextR = mk(b.z, b.w);
     bool cvL = fl & 1, cvR = (fl >> 1) & 1;
@@ -703,8 +707,8 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
 
 // The 64 BF16 features of one agent (layout in mlp.h) to out[0..7].
 template <int K>
-__device__ void agent_features(const DevState& s, int t, V2 position, V2 velocity, float rad, float2 pv, int cnt,
-                               const Near4& T, uint4* out) {
+__device__ void agent_features(const DevState& s, const EnvDesc& E, int t, V2 position, V2 velocity, float rad,
+                               float2 pv, int cnt, const Near4& T, uint4* out) {
     float f[mlp::kInPad];
 #pragma unroll
     for (int i = 0; i < mlp::kInPad; ++i) f[i] = 0.0f;
@@ -723,7 +727,7 @@ __device__ void agent_features(const DevState& s, int t, V2 position, V2 velocit
 #pragma unroll
     for (int i = 0; i < mlp::kObstFeat; ++i) {
         if (T.e[i] != INT_MAX) {
-            const float4 a = s.oe0[T.e[i]];
+            const float4 a = E.oe0[T.e[i]];
             f[40 + 4 * i + 0] = feat_clamp((a.x - position.x) * mlp::kPosScale);
             f[40 + 4 * i + 1] = feat_clamp((a.y - position.y) * mlp::kPosScale);
             f[40 + 4 * i + 2] = feat_clamp((a.z - position.x) * mlp::kPosScale);
@@ -745,7 +749,7 @@ __device__ void agent_features(const DevState& s, int t, V2 position, V2 velocit
 // the agent's bucket; `emit(e, dd)` runs for every edge it keeps, in scan
 // order, and returns false to stop the scan.
 template <typename F>
-__device__ __forceinline__ void obstacle_query(const DevState& s, V2 position, float range2, F&& emit) {
+__device__ __forceinline__ void obstacle_query(const EnvDesc& s, V2 position, float range2, F&& emit) {
     int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
     int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
     int b = by * s.og.side + bx;
@@ -863,17 +867,18 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     const float speed = s.pref_speed[id];
     const float maxSpeed = s.max_speed_from_pref ? speed : s.max_speed;
     const float2 pv = s.pref_vel[id];
+    const EnvDesc& E = env_of(s, id);
 
     // obstacle neighbours in scan order, constraints generated in place
     int nc = 0;
     Near4 T;
     if (MODE != kPlain) near4_init(T);
-    if (s.n_obst_edges > 0) {
+    if (E.n_obst_edges > 0) {
         const float range = s.time_horizon_obst * maxSpeed + rad;          // ORCA.cpp:27
         bool parked = false;
-        obstacle_query(s, position, range * range, [&](int e, float dd) {
+        obstacle_query(E, position, range * range, [&](int e, float dd) {
             V2 cp, cn;
-            const bool made = obstacle_constraint(s, e, position, velocity, rad, cp, cn);
+            const bool made = obstacle_constraint(s, E, e, position, velocity, rad, cp, cn);
             if (made && OVERFLOW && nc == OSLOTS) {
                 s.ovf_agent[wave_alloc(&s.stats[1])] = t;
                 parked = true;
@@ -949,7 +954,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         // training sample: features, active slots at the LP velocity, v
         const uint32_t idx = wave_alloc(s.log_n);
         if (idx < s.log_cap) {
-            agent_features<K>(s, t, position, velocity, rad, pv, cnt, T, s.log_x + (size_t)idx * 8);
+            agent_features<K>(s, E, t, position, velocity, rad, pv, cnt, T, s.log_x + (size_t)idx * 8);
             uint32_t act = 0, valid = 0;
             bool repr = true;
             for (int c = 0; c < nc; ++c) {
@@ -983,7 +988,7 @@ constexpr int kObstSlots = 6;
 template <int K, int MODE>
 __global__ __launch_bounds__(64) void k_orca(DevState s) {
     __shared__ float4 lds[(kObstSlots + K) * 64];
-    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const uint32_t n_active = s.cell_start[s.n_keys];
     const int tid = threadIdx.x;
     const int t = xcd_block() * 64 + tid;
     if (t >= (int)n_active) return;
@@ -1036,7 +1041,7 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
 // destroyed this step).
 template <int K>
 __global__ __launch_bounds__(256) void k_features(DevState s) {
-    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
     const float4 pr = s.sorted_pr[t];
@@ -1052,21 +1057,22 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
     const V2 velocity = mk(v0.x, v0.y);
     const float rad = pr.z;
     const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
+    const EnvDesc& E = env_of(s, id);
     Near4 T;
     near4_init(T);
-    if (s.n_obst_edges > 0) {
+    if (E.n_obst_edges > 0) {
         const float range = s.time_horizon_obst * maxSpeed + rad;
-        obstacle_query(s, position, range * range, [&](int e, float dd) {
+        obstacle_query(E, position, range * range, [&](int e, float dd) {
             near4_insert(T, dd, e, -1);
             return true;
         });
     }
-    agent_features<K>(s, t, position, velocity, rad, s.pref_vel[id], s.nbr_cnt[t], T, out);
+    agent_features<K>(s, E, t, position, velocity, rad, s.pref_vel[id], s.nbr_cnt[t], T, out);
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
 __global__ void k_export_nbrs(DevState s, int* out, int* out_cnt) {
-    const uint32_t n_active = s.cell_start[n_grid_keys(s.ag)];
+    const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
     int id = __float_as_int(s.sorted_pr[t].w);
@@ -1086,7 +1092,7 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 static inline int round8(int n) { return (n + 7) & ~7; }
 
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) {
-    const uint32_t G = 1u << (2 * s.ag.bits);
+    const uint32_t G = s.n_keys;
     hipLaunchKernelGGL(k_grid_count, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
     const uint32_t nb = (G + 1023u) / 1024u;
     hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, st, (const uint32_t*)s.cell_count, s.cell_start, bsum, G);

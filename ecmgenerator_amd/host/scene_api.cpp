@@ -131,7 +131,7 @@ extern "C" int ecm_scene_plan_path(ecm_scene* s, float sx, float sy, float gx, f
 extern "C" int ecm_scene_plan_paths(ecm_scene* s, int n, const float* st, const float* go, const float* clr,
                                     int threads, int* cnt, float* pts, long long cap, int* off, int) {
     if (!s || n < 0 || (n > 0 && (!st || !go || !clr || !cnt || !pts || !off))) return sfail(ECM_E_ARG, "null argument");
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     std::vector<std::vector<float>> res(n);
     std::atomic<int> next{0};
     std::vector<std::thread> th;
@@ -201,9 +201,23 @@ void add_rect(Scene& sc, const RectD& r) {
 
 }  // namespace
 
+static int generate(Cfg cf, uint64_t seed, int n_agents, ecm_scene** scene_out, ecm_scenario** out);
+
 extern "C" int ecm_scenario_generate(int config, uint64_t seed, int n_agents, ecm_scene** scene_out, ecm_scenario** out) {
     if (!scene_out || !out) return sfail(ECM_E_ARG, "null argument");
-    Cfg cf = config_of(config);
+    return generate(config_of(config), seed, n_agents, scene_out, out);
+}
+
+extern "C" int ecm_scenario_generate_box(float half, int n_rects, int side_lo, int side_hi, int n_agents, uint64_t seed,
+                                         ecm_scene** scene_out, ecm_scenario** out) {
+    if (!scene_out || !out) return sfail(ECM_E_ARG, "null argument");
+    if (!(half >= 10.0f) || n_rects < 0 || side_lo < 1 || side_hi < side_lo || n_agents < 1)
+        return sfail(ECM_E_ARG, "bad scenario arguments");
+    Cfg cf{(double)half, n_rects, side_lo, side_hi, n_agents, false};
+    return generate(cf, seed, n_agents, scene_out, out);
+}
+
+static int generate(Cfg cf, uint64_t seed, int n_agents, ecm_scene** scene_out, ecm_scenario** out) {
     if (n_agents <= 0) n_agents = cf.n_agents;
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<double> U(0.0, 1.0);
@@ -300,7 +314,7 @@ extern "C" int ecm_scenario_generate(int config, uint64_t seed, int n_agents, ec
         std::vector<std::vector<float>> paths(m);
         std::atomic<int> next{0};
         std::vector<std::thread> th;
-        int nt = (int)std::max(1u, std::thread::hardware_concurrency());
+        int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
         for (int t = 0; t < nt; ++t)
             th.emplace_back([&]() {
                 for (int i; (i = next++) < m;)

@@ -463,7 +463,7 @@ int Scene::build_ecm(double h, int threads) {
             report.message = "obstacles must lie strictly inside the walkable box";
             return -1;
         }
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     threads = std::min(threads, 64);
     ensure_rgrid();
     Builder B(*this);

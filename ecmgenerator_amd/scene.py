@@ -81,6 +81,20 @@ def generate(config: int, seed: int = 1, n_agents: int = 0):
     L = N.lib()
     sh, ah = C.c_void_p(), C.c_void_p()
     N.check(L.ecm_scenario_generate(int(config), C.c_uint64(seed), int(n_agents), C.byref(sh), C.byref(ah)))
+    half = {0: 100.0, 1: 500.0, 2: 1000.0}.get(config, 30.0)
+    return _export(L, sh, ah, half)
+
+
+def generate_box(half: float, n_rects: int, side_lo: int, side_hi: int, n_agents: int, seed: int):
+    """A square scenario of the given size (ecm_scenario_generate_box)."""
+    L = N.lib()
+    sh, ah = C.c_void_p(), C.c_void_p()
+    N.check(L.ecm_scenario_generate_box(float(half), int(n_rects), int(side_lo), int(side_hi), int(n_agents),
+                                        C.c_uint64(seed), C.byref(sh), C.byref(ah)))
+    return _export(L, sh, ah, float(half))
+
+
+def _export(L, sh, ah, half):
     n, npts = C.c_int(), C.c_longlong()
     N.check(L.ecm_scenario_counts(ah, C.byref(n), C.byref(npts)))
     st = np.zeros((n.value, 2), np.float32)
@@ -93,6 +107,5 @@ def generate(config: int, seed: int = 1, n_agents: int = 0):
     N.check(L.ecm_scenario_export(ah, N.fptr(st), N.fptr(go), N.fptr(r), N.fptr(s), N.iptr(off), N.iptr(cnt),
                                   N.fptr(path)))
     L.ecm_scenario_destroy(ah)
-    half = {0: 100.0, 1: 500.0, 2: 1000.0}.get(config, 30.0)
     scene = Scene(box=(-half, -half, half, half), handle=sh)
     return scene, Agents(st, r, s, off, cnt, path[: npts.value]), go

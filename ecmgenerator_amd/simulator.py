@@ -137,28 +137,54 @@ class Simulator:
     def Initialize(self) -> None:        # Simulator.cpp:21-60 (buffers exist after create)
         return None
 
-    def SetObstacles(self, obs: Obstacles) -> None:
+    def SetObstacles(self, obs: Obstacles, env: int = 0) -> None:
         xy = np.ascontiguousarray(obs.xy, dtype=np.float32)
         ro = np.ascontiguousarray(obs.ring_off, dtype=np.int32)
-        N.check(self._L.ecm_sim_set_obstacles(self._h, N.fptr(xy), N.iptr(ro), obs.n_rings))
+        N.check(self._L.ecm_sim_env_set_obstacles(self._h, int(env), N.fptr(xy), N.iptr(ro), obs.n_rings))
 
-    def SetEcm(self, ecm: Ecm) -> None:
+    def SetEcm(self, ecm: Ecm, env: int = 0) -> None:
         v = np.ascontiguousarray(ecm.vert_xy, dtype=np.float32)
         t = np.ascontiguousarray(ecm.he_target, dtype=np.int32)
         p = np.ascontiguousarray(ecm.he_pts, dtype=np.float32)
-        N.check(self._L.ecm_sim_set_ecm(self._h, ecm.n_vert, N.fptr(v), ecm.n_edge, N.iptr(t), N.fptr(p)))
+        N.check(self._L.ecm_sim_env_set_ecm(self._h, int(env), ecm.n_vert, N.fptr(v), ecm.n_edge, N.iptr(t),
+                                            N.fptr(p)))
 
-    def AttachPlanner(self, scene) -> None:
+    def AttachPlanner(self, scene, env: int = 0) -> None:
         """UpdatePath on IRM failure (Simulator.cpp:579-587) through a scene's planner."""
         fn = self._L.ecm_scene_replan_callback()
-        N.check(self._L.ecm_sim_set_replan_callback(self._h, fn, scene.handle))
+        N.check(self._L.ecm_sim_env_set_replan_callback(self._h, int(env), fn, scene.handle))
         # the scene planner is thread-safe (read-only ECM and index)
         N.check(self._L.ecm_sim_set_replan_threads(self._h, max(1, min(16, os.cpu_count() or 1))))
-        self._planner = scene
+        self._planners = getattr(self, "_planners", {})
+        self._planners[int(env)] = scene
+        if env == 0:
+            self._planner = scene
+
+    # ---- environments (batched multi-scenario stepping) -------------------
+    def AddEnvironment(self, ecm: Ecm, obstacles: Obstacles, world, planner=None) -> int:
+        """A further independent scene advanced by the same step; returns its id."""
+        e = C.c_int()
+        N.check(self._L.ecm_sim_add_env(self._h, *[float(v) for v in world], C.byref(e)))
+        self.SetObstacles(obstacles, e.value)
+        self.SetEcm(ecm, e.value)
+        if planner is not None:
+            self.AttachPlanner(planner, e.value)
+        return e.value
+
+    def num_environments(self) -> int:
+        n = C.c_int()
+        N.check(self._L.ecm_sim_num_envs(self._h, C.byref(n)))
+        return n.value
+
+    def agent_environments(self) -> np.ndarray:
+        out = np.empty(self.max_agents, np.int32)
+        N.check(self._L.ecm_sim_agent_envs(self._h, N.iptr(out)))
+        return out
 
     # ---- agents ---------------------------------------------------------
-    def SpawnAgents(self, agents: Agents, validate: bool = True) -> np.ndarray:
-        """SpawnAgent (Simulator.cpp:168-200) for a batch; -1 where rejected."""
+    def SpawnAgents(self, agents: Agents, validate: bool = True, env: int = 0) -> np.ndarray:
+        """SpawnAgent (Simulator.cpp:168-200) for a batch into environment
+        `env`; -1 where rejected."""
         n = agents.n
         ids = np.full(n, -1, dtype=np.int32)
         st = np.ascontiguousarray(agents.start, dtype=np.float32)
@@ -167,8 +193,8 @@ class Simulator:
         po = np.ascontiguousarray(agents.path_off, dtype=np.int32)
         pc = np.ascontiguousarray(agents.path_cnt, dtype=np.int32)
         px = np.ascontiguousarray(agents.path_xy, dtype=np.float32)
-        N.check(self._L.ecm_sim_spawn(self._h, n, N.fptr(st), N.fptr(r), N.fptr(s), N.iptr(po), N.iptr(pc),
-                                      N.fptr(px), int(validate), N.iptr(ids)))
+        N.check(self._L.ecm_sim_env_spawn(self._h, int(env), n, N.fptr(st), N.fptr(r), N.fptr(s), N.iptr(po),
+                                          N.iptr(pc), N.fptr(px), int(validate), N.iptr(ids)))
         return ids
 
     def SpawnAgent(self, start, goal_path: Sequence, clearance: float, preferred_speed: float) -> int:

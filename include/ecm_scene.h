@@ -65,6 +65,11 @@ ecm_replan_fn ecm_scene_replan_callback(void);
 typedef struct ecm_scenario ecm_scenario;
 int ecm_scenario_generate(int config, uint64_t seed, int n_agents, ecm_scene** scene_out,
                           ecm_scenario** out);
+/* The same generator for a (2 half) x (2 half) m square with n_rects
+ * rectangles of integer sides in [side_lo, side_hi] m and n_agents agents
+ * (one environment of the batched multi-scenario workload). */
+int ecm_scenario_generate_box(float half, int n_rects, int side_lo, int side_hi, int n_agents, uint64_t seed,
+                              ecm_scene** scene_out, ecm_scenario** out);
 int ecm_scenario_counts(ecm_scenario* sc, int* n_agents, long long* n_points);
 int ecm_scenario_export(ecm_scenario* sc, float* start_xy, float* goal_xy, float* radius,
                         float* pref_speed, int* path_off, int* path_cnt, float* path_xy);

@@ -98,6 +98,26 @@ int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user);
  * (default 1).  Only raise it for a thread-safe callback, such as
  * ecm_scene_replan_callback(). */
 int ecm_sim_set_replan_threads(ecm_sim* s, int n);
+
+/* Batched multi-scenario stepping.  A handle holds one or more
+ * environments: independent scenes (world box, obstacles, ECM, agents)
+ * whose agents never interact across environments, advanced together by
+ * the same launches of every step.  ecm_sim_create makes environment 0;
+ * the calls without an environment argument act on it.  Agents keep their
+ * environment for life; slot ids are shared by all environments.
+ * (No reference counterpart: the reference simulates one environment,
+ * Simulator.h:59-188.) */
+int ecm_sim_add_env(ecm_sim* s, float x0, float y0, float x1, float y1, int* env);
+int ecm_sim_num_envs(ecm_sim* s, int* n);
+int ecm_sim_env_set_obstacles(ecm_sim* s, int env, const float* xy, const int* ring_off, int n_rings);
+int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float* vert_xy, int n_edge, const int* he_target,
+                        const float* he_pts);
+int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_xy, const float* radius,
+                      const float* pref_speed, const int* path_off, const int* path_cnt, const float* path_xy,
+                      int validate, int* out_ids);
+int ecm_sim_env_set_replan_callback(ecm_sim* s, int env, ecm_replan_fn fn, void* user);
+/* Environment of every slot (max_agents entries, -1 for free slots). */
+int ecm_sim_agent_envs(ecm_sim* s, int* env);
 int ecm_sim_step(ecm_sim* s, int n_steps);
 /* Wait for the handle's stream. */
 int ecm_sim_sync(ecm_sim* s);
