@@ -32,7 +32,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=None)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample-agents", type=int, default=400)
+    ap.add_argument("--cpu-sample-agents", type=int, default=40000,
+                    help="agents in the CPU baseline's sample (about 10 s of single-core work at 1M agents)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for HBM counters)")
@@ -40,6 +41,9 @@ def parse():
                     help="proposal-network weights (.npz) for the learned leg; default: the committed weights if present")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--theta", type=float, default=None, help="proposal logit threshold (default: from the weights file)")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of replaying step graphs")
+    ap.add_argument("--autotune", type=int, default=3,
+                    help="steps per autotuner trial during warmup (0: default launch shape)")
     return ap.parse_args()
 
 
@@ -71,13 +75,24 @@ def main():
     t0 = time.time()
     wl = W.build(name, seed=args.seed)
     t_setup = time.time() - t0
-    print(f"[bench] workload {name}: {wl.agents.n} agents, {wl.ecm.n_edge} ECM edges, "
-          f"{wl.obstacles.n_rings} obstacles, setup {t_setup:.1f}s", file=sys.stderr, flush=True)
-    sim = Simulator(wl.ecm, wl.obstacles, wl.box, wl.params, planner=wl.scene)
-    ids = sim.SpawnAgents(wl.agents)
-    n_spawned = int((ids >= 0).sum())
+    batched = isinstance(wl, W.BatchWorkload)
+    if batched:
+        print(f"[bench] workload {name}: {len(wl.envs)} environments, {wl.n_agents} agents, setup {t_setup:.1f}s",
+              file=sys.stderr, flush=True)
+        sim, ids_per_env = wl.make_sim()
+        n_spawned = int(sum((i >= 0).sum() for i in ids_per_env))
+        cnts = np.concatenate([w.agents.path_cnt[i >= 0] for w, i in zip(wl.envs, ids_per_env)])
+        mean_path = float(cnts.mean())
+    else:
+        print(f"[bench] workload {name}: {wl.agents.n} agents, {wl.ecm.n_edge} ECM edges, "
+              f"{wl.obstacles.n_rings} obstacles, setup {t_setup:.1f}s", file=sys.stderr, flush=True)
+        sim = Simulator(wl.ecm, wl.obstacles, wl.box, wl.params, planner=wl.scene)
+        ids = sim.SpawnAgents(wl.agents)
+        n_spawned = int((ids >= 0).sum())
+        mean_path = float(wl.agents.path_cnt[ids >= 0].mean())
     print(f"[bench] spawned {n_spawned}", file=sys.stderr, flush=True)
-    mean_path = float(wl.agents.path_cnt[ids >= 0].mean())
+    if not args.no_graph:
+        sim.set_graph(True)
 
     for _ in range(args.warmup):
         sim.Update()
@@ -86,6 +101,12 @@ def main():
     snap_dir = tempfile.mkdtemp(prefix="ecm_bench_")
     snap = os.path.join(snap_dir, "state.bin")
     sim.snapshot_save(snap)
+    # the autotuner times its trials on the live workload (advancing it);
+    # every timed step below starts again from the state after warmup
+    tuning = None
+    if args.autotune > 0:
+        tuning = sim.autotune(args.autotune)
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
     # both legs time the same steps: the state just loaded, one untimed step
     # (buffer re-creation after the load), then K timed steps
     first_ms = _first_step_after_load(sim, snap)
@@ -124,6 +145,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": names[dom],
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
+    graph_on, graph_why = sim.graph_status()
     learned = None
     wpath = args.learned or _default_weights()
     if not args.no_learned and wpath and os.path.exists(wpath):
@@ -136,11 +158,11 @@ def main():
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb)
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not batched:
         cpu = cpu_baseline(sim, wl, args.cpu_sample_agents)
     sim.close()
     line = {
-        "metric": "agent-updates/s (plain GPU ORCA step)",
+        "metric": "agent-updates/s at 1M agents, learned vs plain ORCA at 1e-4 m/s; % HBM peak",
         "value": round(value, 1),
         "unit": "agent-updates/s",
         "n_gpus": 1,
@@ -154,6 +176,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": name, **wl.describe(), "agents_spawned": n_spawned,
                    "mean_path_points": round(mean_path, 2)},
+        "launch": {"step_graphs": graph_on, "graph_fallback_reason": graph_why or None, "autotune": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned": learned,

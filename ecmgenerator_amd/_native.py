@@ -32,6 +32,12 @@ class SimParams(C.Structure):
     ]
 
 
+class SimTuning(C.Structure):
+    """ecm_sim_tuning (include/ecm_sim.h)."""
+    _fields_ = [("cell_scale", C.c_float), ("knn_block", C.c_int), ("orca_obst_slots", C.c_int),
+                ("best_ms", C.c_float)]
+
+
 REPLAN_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                         C.c_float, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int))
 
@@ -67,6 +73,11 @@ SIGNATURES = {
     "ecm_sim_get_replans": (_I, [_P, _PI, _I, _PI]),
     "ecm_sim_read_paths": (_I, [_P, _PI, _PI, _PF, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ecm_sim_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
+    "ecm_sim_set_graph": (_I, [_P, _I]),
+    "ecm_sim_graph_status": (_I, [_P, _PI, C.c_char_p, _I]),
+    "ecm_sim_get_tuning": (_I, [_P, C.POINTER(SimTuning)]),
+    "ecm_sim_set_tuning": (_I, [_P, C.POINTER(SimTuning)]),
+    "ecm_sim_autotune": (_I, [_P, _I, C.POINTER(SimTuning)]),
     "ecm_sim_add_env": (_I, [_P, _F, _F, _F, _F, _PI]),
     "ecm_sim_num_envs": (_I, [_P, _PI]),
     "ecm_sim_env_set_obstacles": (_I, [_P, _I, _PF, _PI, _I]),

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <chrono>
 #include <memory>
 #include <string>
 #include <thread>
@@ -22,8 +23,8 @@
 
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
-hipError_t launch_attr_knn(const DevState& s, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, int mode, hipStream_t st);
+hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
@@ -188,6 +189,28 @@ struct ecm_sim {
     DBuf<uint4> feat;
     DBuf<uint32_t> mlp_mask;
     ecm_mlp* log_mlp = nullptr;     // set for the duration of ecm_sim_log_step
+
+    // launch shape (results do not depend on it; ecm_sim_autotune picks it)
+    ecm_sim_tuning tune{1.0f, 256, 6, 0.0f};
+    // the device part of a step captured as two hipGraphs (up to the replan
+    // read-back, and the rest), one pair per position buffer parity,
+    // recaptured when anything they were captured with changes
+    struct GraphKey {
+        DevState d;
+        int maxo, mode, oslots, knn_block, n_cu;
+        float theta;
+        const void* frags;
+        const void* pbias;
+    };
+    struct StepGraph {
+        bool valid = false;
+        GraphKey key;
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+    };
+    bool use_graph = false;
+    StepGraph graphs[2];
+    std::string graph_error;
     std::vector<int> h_replans;
 };
 
@@ -237,6 +260,14 @@ static void free_all(ecm_sim* s) {
     s->replan_pos.release();
     s->feat.release();
     s->mlp_mask.release();
+    for (auto& g : s->graphs)
+        for (int h = 0; h < 2; ++h) {
+            if (g.exec[h]) (void)hipGraphExecDestroy(g.exec[h]);
+            if (g.graph[h]) (void)hipGraphDestroy(g.graph[h]);
+            g.exec[h] = nullptr;
+            g.graph[h] = nullptr;
+            g.valid = false;
+        }
     for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (s->ev_replan) (void)hipEventDestroy(s->ev_replan);
     s->ev_replan = nullptr;
@@ -900,7 +931,7 @@ static int ensure_agent_grid(ecm_sim* s) {
         float cell = s->prm.grid_cell;
         if (cell <= 0.0f) {
             double area = (double)(e->x1 - e->x0) * (e->y1 - e->y0);
-            cell = (float)std::sqrt(area / n);
+            cell = (float)std::sqrt(area / n) * s->tune.cell_scale;
             cell = std::min(std::max(cell, 0.5f), 32.0f);
         }
         e->ag = make_grid(e->x0, e->y0, e->x1, e->y1, cell, 2048);
@@ -1065,7 +1096,9 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     return 0;
 }
 
-static int step_once(ecm_sim* s) {
+// Host work before a step: agent grids, obstacle buckets, environment
+// descriptors and queue capacity; then the kernel arguments.
+static int prepare_step(ecm_sim* s, DevState* out) {
     // UpdateMaxAgentIndex (Simulator.cpp:481-492)
     while (s->last_idx >= 0 && !s->h_active[s->last_idx]) --s->last_idx;
     int rc = ensure_agent_grid(s);
@@ -1097,33 +1130,121 @@ static int step_once(ecm_sim* s) {
             s->q3d_stride = stride;
         }
     }
-    DevState d = make_state(s);
-    // destroyed / replan lists and every per-step counter restart
-    HIPCHK(launch_fill_u32(s->counters.p, 0u, ecm_sim::kNCounters, s->stream));
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
-    HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
-    HIPCHK(launch_attr_knn(d, s->stream));
-    // replan requests are final once k_attr_knn is done: read them back
-    // and plan on the host while k_orca runs
-    const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
-    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipMemcpyAsync(s->h_replan_ids, s->replan_list.p, peek * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipEventRecord(s->ev_replan, s->stream));
+    *out = make_state(s);
+    return 0;
+}
+
+static int step_mode(const ecm_sim* s) { return s->log_mlp ? 1 : (s->mlp ? 2 : 0); }
+
+// The device operations of one step, in stream order, in two halves: up to
+// the replan read-back (half 0) and the rest (half 1).  No memset and no
+// host synchronization: each half is what one step graph captures.
+static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half) {
+    if (half == 0) {
+        // destroyed / replan lists and every per-step counter restart
+        HIPCHK(launch_fill_u32(s->counters.p, 0u, ecm_sim::kNCounters, s->stream));
+        if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
+        HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
+        if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
+        HIPCHK(launch_attr_knn(d, s->tune.knn_block, s->stream));
+        // replan requests are final once k_attr_knn is done: read them back
+        // and plan on the host while k_orca runs
+        const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
+        HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipMemcpyAsync(s->h_replan_ids, s->replan_list.p, peek * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+        return 0;
+    }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
-    int mode = s->log_mlp ? 1 : 0;
-    if (s->mlp && !s->log_mlp) {
+    if (mode == 2) {
         // learned proposal: features, network, then the verifying ORCA pass
-        mode = 2;
         HIPCHK(launch_features(d, s->stream));
-        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + s->n_keys, 0, s->mlp->frags.p,
-                                     s->mlp->pbias.p, s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu,
-                                     s->stream));
+        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + s->n_keys, 0, s->mlp->frags.p, s->mlp->pbias.p,
+                                     s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu, s->stream));
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, mode, s->stream));
+    HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, ecm_sim::kNCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s->stream));
+    return 0;
+}
+
+static void drop_graph(ecm_sim::StepGraph& g) {
+    for (int h = 0; h < 2; ++h) {
+        if (g.exec[h]) (void)hipGraphExecDestroy(g.exec[h]);
+        if (g.graph[h]) (void)hipGraphDestroy(g.graph[h]);
+        g.exec[h] = nullptr;
+        g.graph[h] = nullptr;
+    }
+    g.valid = false;
+}
+
+// Captures both halves of the step for this buffer parity when missing or
+// stale.  Returns 1 when capture is not possible, after recording why in
+// graph_error.
+static int ensure_step_graph(ecm_sim* s, const DevState& d, int mode) {
+    ecm_sim::GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.d = d;
+    key.maxo = s->maxo;
+    key.mode = mode;
+    key.oslots = s->tune.orca_obst_slots;
+    key.knn_block = s->tune.knn_block;
+    key.theta = s->mlp_theta;
+    key.frags = mode == 2 ? (const void*)s->mlp->frags.p : nullptr;
+    key.pbias = mode == 2 ? (const void*)s->mlp->pbias.p : nullptr;
+    key.n_cu = mode == 2 ? s->mlp->n_cu : 0;
+    ecm_sim::StepGraph& g = s->graphs[s->cur];
+    if (g.valid && std::memcmp(&g.key, &key, sizeof(key)) == 0) return 0;
+    drop_graph(g);
+    for (int h = 0; h < 2; ++h) {
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            s->graph_error = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
+            drop_graph(g);
+            return 1;
+        }
+        int rc = enqueue_device_half(s, d, mode, h);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(s->stream, &graph);
+        hipGraphExec_t exec = nullptr;
+        if (!rc && e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (rc || e != hipSuccess || !graph || !exec) {
+            if (graph) (void)hipGraphDestroy(graph);
+            s->graph_error = rc ? std::string("capture: ") + ecm_last_error()
+                                : std::string("graph capture / instantiate: ") + hipGetErrorString(e);
+            (void)hipGetLastError();
+            drop_graph(g);
+            return 1;
+        }
+        g.graph[h] = graph;
+        g.exec[h] = exec;
+    }
+    g.key = key;
+    g.valid = true;
+    return 0;
+}
+
+static int step_once(ecm_sim* s) {
+    DevState d;
+    int rc = prepare_step(s, &d);
+    if (rc) return rc;
+    const int mode = step_mode(s);
+    bool graph = false;
+    if (s->use_graph && !s->timing) {
+        rc = ensure_step_graph(s, d, mode);
+        if (rc < 0) return rc;
+        if (rc == 0) graph = true;
+        else s->use_graph = false;           // capture refused: eager from now on (see graph_error)
+    }
+    // half 0, the replan read-back event, half 1
+    if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur].exec[0], s->stream));
+    else if ((rc = enqueue_device_half(s, d, mode, 0)) != 0) return rc;
+    HIPCHK(hipEventRecord(s->ev_replan, s->stream));
+    if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur].exec[1], s->stream));
+    else if ((rc = enqueue_device_half(s, d, mode, 1)) != 0) return rc;
+    const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
     HIPCHK(hipEventSynchronize(s->ev_replan));
     const uint32_t nr = s->h_counters[1];
     std::vector<int> rid(s->h_replan_ids, s->h_replan_ids + std::min<uint32_t>(nr, peek));
@@ -1131,9 +1252,9 @@ static int step_once(ecm_sim* s) {
     if (nr > (uint32_t)peek) {
         rid.resize(nr);
         rpos.resize(nr);
-        HIPCHK(hipMemcpyAsync(rid.data() + peek, s->replan_list.p + peek, (nr - peek) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipMemcpyAsync(rpos.data() + peek, s->replan_pos.p + peek, (nr - peek) * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(rid.data() + peek, s->replan_list.p + peek, (nr - peek) * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rpos.data() + peek, s->replan_pos.p + peek, (nr - peek) * sizeof(float2), hipMemcpyDeviceToHost));
     }
     // order as the reference visits slots (Simulator.cpp:545)
     {
@@ -1153,8 +1274,6 @@ static int step_once(ecm_sim* s) {
         rc = run_replans(s, rid, rpos);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, ecm_sim::kNCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
     const int n_before = s->num_agents;
@@ -1196,6 +1315,100 @@ extern "C" int ecm_sim_step(ecm_sim* s, int n_steps) {
         int rc = step_once(s);
         if (rc) return rc;
     }
+    return 0;
+}
+
+extern "C" int ecm_sim_set_graph(ecm_sim* s, int on) {
+    if (!s) return fail(ECM_E_ARG, "null handle");
+    s->use_graph = on != 0;
+    if (on) s->graph_error.clear();
+    return 0;
+}
+
+extern "C" int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap) {
+    if (!s || !active) return fail(ECM_E_ARG, "null argument");
+    *active = s->use_graph ? 1 : 0;
+    if (why && cap > 0) {
+        std::strncpy(why, s->graph_error.c_str(), (size_t)cap - 1);
+        why[cap - 1] = 0;
+    }
+    return 0;
+}
+
+static int check_tuning(const ecm_sim_tuning& t) {
+    if (!(t.cell_scale >= 0.25f && t.cell_scale <= 4.0f)) return fail(ECM_E_ARG, "cell_scale must be in [0.25, 4]");
+    if (t.knn_block != 64 && t.knn_block != 128 && t.knn_block != 256) return fail(ECM_E_ARG, "knn_block must be 64, 128 or 256");
+    if (t.orca_obst_slots != 4 && t.orca_obst_slots != 6 && t.orca_obst_slots != 8)
+        return fail(ECM_E_ARG, "orca_obst_slots must be 4, 6 or 8");
+    return 0;
+}
+
+static void apply_tuning(ecm_sim* s, const ecm_sim_tuning& t) {
+    if (t.cell_scale != s->tune.cell_scale)
+        for (auto& e : s->envs) e->grid_for_n = -1;     // new agent grids at the next step
+    const float best = s->tune.best_ms;
+    s->tune = t;
+    s->tune.best_ms = best;
+}
+
+extern "C" int ecm_sim_get_tuning(ecm_sim* s, ecm_sim_tuning* out) {
+    if (!s || !out) return fail(ECM_E_ARG, "null argument");
+    *out = s->tune;
+    return 0;
+}
+
+extern "C" int ecm_sim_set_tuning(ecm_sim* s, const ecm_sim_tuning* t) {
+    if (!s || !t) return fail(ECM_E_ARG, "null argument");
+    int rc = check_tuning(*t);
+    if (rc) return rc;
+    apply_tuning(s, *t);
+    return 0;
+}
+
+// Coordinate search over the launch shape on the live workload: the agent
+// grid cell scale, then the k-nearest block size, then the obstacle slots
+// of the main ORCA tile.  Every trial runs one untimed step (rebuilds) and
+// `steps` timed steps; the simulation advances through all of them.
+extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
+    if (!s || steps < 1) return fail(ECM_E_ARG, "bad autotune arguments");
+    for (size_t e = 0; e < s->envs.size(); ++e)
+        if (!s->envs[e]->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before autotune (environment %zu)", e);
+    (void)hipSetDevice(s->dev);
+    auto trial = [&](const ecm_sim_tuning& t, float* ms) -> int {
+        apply_tuning(s, t);
+        int rc = step_once(s);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps; ++i)
+            if ((rc = step_once(s)) != 0) return rc;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        return 0;
+    };
+    ecm_sim_tuning best = s->tune;
+    float best_ms = 0.0f;
+    int rc = trial(best, &best_ms);
+    if (rc) return rc;
+    const float scales[] = {0.6f, 0.8f, 1.0f, 1.25f, 1.6f, 2.0f};
+    const int blocks[] = {64, 128, 256};
+    const int slots[] = {4, 6, 8};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int nopt = dim == 0 ? 6 : 3;
+        for (int o = 0; o < nopt; ++o) {
+            ecm_sim_tuning t = best;
+            if (dim == 0) t.cell_scale = scales[o];
+            else if (dim == 1) t.knn_block = blocks[o];
+            else t.orca_obst_slots = slots[o];
+            if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;
+            float ms = 0.0f;
+            if ((rc = trial(t, &ms)) != 0) return rc;
+            if (ms < best_ms) { best_ms = ms; best = t; }
+        }
+    }
+    apply_tuning(s, best);
+    s->tune.best_ms = best_ms;
+    if (out) *out = s->tune;
     return 0;
 }
 

@@ -983,19 +983,19 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
 }
 
 // Main ORCA pass: one agent per lane in Morton order, a small constraint
-// tile (kObstSlots obstacle slots) so that more waves fit per CU.
-constexpr int kObstSlots = 6;
-template <int K, int MODE>
+// tile (OSLOTS obstacle slots: 4, 6 or 8, picked by the autotuner) so that
+// more waves fit per CU.
+template <int K, int MODE, int OSLOTS>
 __global__ __launch_bounds__(64) void k_orca(DevState s) {
-    __shared__ float4 lds[(kObstSlots + K) * 64];
+    __shared__ float4 lds[(OSLOTS + K) * 64];
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int tid = threadIdx.x;
     const int t = xcd_block() * 64 + tid;
     if (t >= (int)n_active) return;
-    orca_agent<K, kObstSlots, true, MODE>(s, t, lds, tid);
+    orca_agent<K, OSLOTS, true, MODE>(s, t, lds, tid);
 }
 
-// Agents with more than kObstSlots obstacle constraints, with a tile for
+// Agents with more obstacle constraints than the main tile holds, with a tile for
 // MAXO (>= the longest obstacle bucket list), plain LP.  Fixed grid,
 // strided loop.
 template <int K, int MAXO>
@@ -1105,12 +1105,19 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
 // Queue kernels run on a fixed grid: their queues hold a few percent of
 // the agents, and a full-size grid would reserve their LDS for nothing.
 constexpr int kQueueBlocks = 1024;
+template <int K, int OSLOTS>
+static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStream_t st) {
+    if (mode == kLearned) hipLaunchKernelGGL((k_orca<K, kLearned, OSLOTS>), g, b, 0, st, s);
+    else if (mode == kLog) hipLaunchKernelGGL((k_orca<K, kLog, OSLOTS>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_orca<K, kPlain, OSLOTS>), g, b, 0, st, s);
+}
+
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, int mode, hipStream_t st) {
+static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, hipStream_t st) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
-    if (mode == kLearned) hipLaunchKernelGGL((k_orca<K, kLearned>), g, b, 0, st, s);
-    else if (mode == kLog) hipLaunchKernelGGL((k_orca<K, kLog>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_orca<K, kPlain>), g, b, 0, st, s);
+    if (oslots <= 4) launch_orca_main<K, 4>(s, mode, g, b, st);
+    else if (oslots <= 6) launch_orca_main<K, 6>(s, mode, g, b, st);
+    else launch_orca_main<K, 8>(s, mode, g, b, st);
     if (maxo <= 16) {
         hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 16>), gq, b, 0, st, s);
@@ -1123,8 +1130,9 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, hipStream_t st)
     }
 }
 
-hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
-    dim3 g(round8(cdiv(s.n_slots, 256))), b(256);
+hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
+    if (block != 64 && block != 128) block = 256;
+    dim3 g(round8(cdiv(s.n_slots, block))), b(block);
     if (s.k == 5) hipLaunchKernelGGL((k_attr_knn<5, true>), g, b, 0, st, s);
     else if (s.k == 10) hipLaunchKernelGGL((k_attr_knn<10, true>), g, b, 0, st, s);
     else if (s.k < 5) hipLaunchKernelGGL((k_attr_knn<5, false>), g, b, 0, st, s);
@@ -1136,10 +1144,10 @@ hipError_t launch_attr_knn(const DevState& s, hipStream_t st) {
 
 // mode: 0 plain, 1 plain + sample log, 2 learned proposal (needs the
 // predicted masks of launch_features + the proposal network)
-hipError_t launch_orca(const DevState& s, int maxo, int mode, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, st);
-    else launch_orca_k<16>(s, maxo, mode, st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, oslots, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, oslots, st);
+    else launch_orca_k<16>(s, maxo, mode, oslots, st);
     return hipGetLastError();
 }
 

@@ -294,6 +294,35 @@ class Simulator:
         N.check(self._L.ecm_sim_last_timing(self._h, a, 4))
         return list(a)
 
+    def set_graph(self, on: bool = True) -> None:
+        """Replay the device part of each step as captured hipGraphs."""
+        N.check(self._L.ecm_sim_set_graph(self._h, int(on)))
+
+    def graph_status(self):
+        """(graphs in use, reason they are not)."""
+        a = C.c_int()
+        why = C.create_string_buffer(512)
+        N.check(self._L.ecm_sim_graph_status(self._h, C.byref(a), why, 512))
+        return bool(a.value), why.value.decode()
+
+    def tuning(self) -> dict:
+        t = N.SimTuning()
+        N.check(self._L.ecm_sim_get_tuning(self._h, C.byref(t)))
+        return {"cell_scale": t.cell_scale, "knn_block": t.knn_block, "orca_obst_slots": t.orca_obst_slots,
+                "best_ms": t.best_ms}
+
+    def set_tuning(self, cell_scale=1.0, knn_block=256, orca_obst_slots=6) -> None:
+        t = N.SimTuning(float(cell_scale), int(knn_block), int(orca_obst_slots), 0.0)
+        N.check(self._L.ecm_sim_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, steps: int = 3) -> dict:
+        """Pick the launch shape by timing trials on this workload (advances
+        the simulation); returns the chosen shape and its step time."""
+        t = N.SimTuning()
+        N.check(self._L.ecm_sim_autotune(self._h, int(steps), C.byref(t)))
+        return {"cell_scale": round(t.cell_scale, 3), "knn_block": t.knn_block, "orca_obst_slots": t.orca_obst_slots,
+                "best_ms": round(t.best_ms, 4)}
+
     def set_learned(self, net, theta: float = 0.0) -> None:
         """Use the learned ORCA proposal of `net` (a learned.ProposalNet), or
         the plain LP with net=None."""

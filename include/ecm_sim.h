@@ -99,6 +99,34 @@ int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user);
  * ecm_scene_replan_callback(). */
 int ecm_sim_set_replan_threads(ecm_sim* s, int n);
 
+/* Step graphs: with on, the device work of a step is captured into
+ * hipGraphs (two per step: up to the replan read-back, and the rest) and
+ * replayed, recaptured whenever buffers, parameters or the launch shape
+ * change.  Steps with timing on run eagerly.  If capture fails the handle
+ * falls back to eager launches; ecm_sim_graph_status reports whether
+ * graphs are in use and, if not, why. */
+int ecm_sim_set_graph(ecm_sim* s, int on);
+int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
+
+/* Launch shape; no choice changes any result.  cell_scale multiplies the
+ * agent grid cell edge (about one agent per cell at 1), knn_block is the
+ * workgroup size of the attraction + k-nearest kernel (64, 128, 256), and
+ * orca_obst_slots the obstacle constraints the main ORCA tile holds (4, 6,
+ * 8; agents needing more take the overflow pass).  best_ms: step time the
+ * autotuner measured (0 before). */
+typedef struct ecm_sim_tuning {
+    float cell_scale;
+    int knn_block;
+    int orca_obst_slots;
+    float best_ms;
+} ecm_sim_tuning;
+int ecm_sim_get_tuning(ecm_sim* s, ecm_sim_tuning* out);
+int ecm_sim_set_tuning(ecm_sim* s, const ecm_sim_tuning* t);
+/* Coordinate search of the launch shape on the current workload, `steps`
+ * timed steps per trial (plus one untimed); the simulation advances
+ * through every trial step.  The best shape is kept and returned. */
+int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out);
+
 /* Batched multi-scenario stepping.  A handle holds one or more
  * environments: independent scenes (world box, obstacles, ECM, agents)
  * whose agents never interact across environments, advanced together by
