@@ -141,8 +141,10 @@ def main():
         "orca_lp_integrate": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
     }
     achieved = bytes_per_agent[names[dom]] * n_per_step / (kt_mean[dom] * 1e-3) / 1e9
+    traffic, tsrc = _profiled_traffic(name, names[dom])
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": names[dom],
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
+            "kernel": names[dom], "algorithmic_bytes_per_launch": int(bytes_per_agent[names[dom]] * n_per_step),
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
     graph_on, graph_why = sim.graph_status()
@@ -183,6 +185,21 @@ def main():
         "stats": {k: st1[k] - st0[k] for k in st1},
     }
     print(json.dumps(line), flush=True)
+
+
+def _profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE / WRITE_SIZE, calibrated against a
+    copy of known size), or (None, None) when there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"<!-- traffic: (\{.*?\}) -->", open(path).read())
+    if not m:
+        return None, None
+    t = json.loads(m.group(1)).get(kernel)
+    return (int(t), os.path.relpath(path, ROOT)) if t else (None, None)
 
 
 def _first_step_after_load(sim, snap):
