@@ -64,3 +64,49 @@ def test_c2_subset_parity_full_size():
     wl = W.build("c2", seed=1)
     worst, st = run(wl, steps=(0, 8), subset_n=600, keep=False)
     assert all(v == 0 for v in worst.values()), worst
+
+
+def test_c2_fp64_lp_resolve_error():
+    """The GPU's ORCA velocities against an FP64 re-solve of the same LPs
+    (same constraints, exact convex optimum) on a sample of c2 agents after
+    the crowd has formed; reports the error distribution and the 3D-LP
+    fallback rate.  Discrete-decision mismatches against the oracle are zero
+    by the bit-exact tests above."""
+    from oracle import oracle_py as O
+    from oracle.lp_fp64 import qp_fp64
+    wl = W.build("c2", seed=1)
+    p = wl.params
+    p.keep_orca_velocity = True
+    sim = Simulator(wl.ecm, wl.obstacles, wl.box, p, planner=wl.scene)
+    sim.SpawnAgents(wl.agents)
+    for _ in range(20):
+        sim.Update()
+    before = oracle_state(sim)
+    st0 = sim.stats()
+    sim.Update()
+    st1 = sim.stats()
+    after = sim.state()
+    nbr, cnt = sim.neighbors()
+    rng = np.random.default_rng(3)
+    act = np.flatnonzero(before["active"].astype(bool) & after["active"].astype(bool))
+    sample = rng.choice(act, size=2000, replace=False)
+    errs, n3d = [], 0
+    for a in sample:
+        cons, nob = O.constraints(p, before, wl.obstacles, int(a), nbr[a][: cnt[a]])
+        vmax = p.max_speed
+        r, _ = O.lp2d(cons, after["pref_velocity"][a], vmax)
+        if r < len(cons):
+            n3d += 1                    # 2D LP infeasible: the 3D LP decides
+            continue
+        opt = qp_fp64(cons, after["pref_velocity"][a], vmax)
+        if opt is None:
+            n3d += 1
+            continue
+        errs.append(float(np.linalg.norm(after["orca_velocity"][a].astype(np.float64) - opt)))
+    e = np.array(errs)
+    lp3d_rate = (st1["lp3d"] - st0["lp3d"]) / max(1, st1["agent_updates"] - st0["agent_updates"])
+    print(f"c2 FP64 re-solve over {e.size} agents: max {e.max():.3e} m/s, p99.9 {np.quantile(e, 0.999):.3e}, "
+          f"median {np.median(e):.3e}; 3D in sample {n3d}/{sample.size}; step 3D-LP rate {lp3d_rate:.4f}", flush=True)
+    assert np.quantile(e, 0.999) <= 1e-4
+    assert np.median(e) <= 1e-6
+    sim.close()

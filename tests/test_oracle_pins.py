@@ -7,6 +7,8 @@ import math
 import os
 
 import numpy as np
+
+from oracle.lp_fp64 import qp_fp64
 import pytest
 
 from ecmgenerator_amd import scenarios as S
@@ -128,40 +130,6 @@ def test_obstacle_halfplane_closed_form(oracle, case):
     assert nob == 1
     assert np.allclose(cs[0, :2], case["normal"], atol=1e-6)
     assert np.allclose(cs[0, 2:], case["point"], atol=1e-6)
-
-
-def qp_fp64(cons, pref, vmax):
-    """min |v - pref|^2 s.t. n.(v - p) >= 0, |v| <= vmax, by enumerating the
-    active sets of size <= 2 (lines and the speed circle)."""
-    cons = np.asarray(cons, np.float64).reshape(-1, 4)
-    N, P = cons[:, :2], cons[:, 2:]
-    pref = np.asarray(pref, np.float64)
-
-    def feas(v, tol=1e-7):
-        return np.linalg.norm(v) <= vmax + tol and np.all(np.einsum("ij,ij->i", N, v - P) >= -tol)
-
-    cands = []
-    q = pref if np.linalg.norm(pref) <= vmax else pref / np.linalg.norm(pref) * vmax
-    cands.append(q)
-    for i in range(len(cons)):
-        n, p = N[i], P[i]
-        d = np.array([n[1], -n[0]])
-        t = d @ (pref - p)
-        cands.append(p + d * t)
-        b = d @ p
-        disc = b * b + vmax * vmax - p @ p
-        if disc >= 0:
-            for s in (-b - math.sqrt(disc), -b + math.sqrt(disc)):
-                cands.append(p + d * s)
-        for j in range(i):
-            A = np.array([N[i], N[j]])
-            if abs(np.linalg.det(A)) < 1e-12:
-                continue
-            cands.append(np.linalg.solve(A, np.array([N[i] @ P[i], N[j] @ P[j]])))
-    feasible = [c for c in cands if feas(c)]
-    if not feasible:
-        return None
-    return min(feasible, key=lambda c: np.linalg.norm(c - pref))
 
 
 def test_lp2d_matches_fp64_qp(oracle):
