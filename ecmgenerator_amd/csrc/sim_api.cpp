@@ -135,6 +135,7 @@ struct ecm_sim {
     DBuf<float4> sorted_pr;
     DBuf<float2> sorted_v;
     DBuf<int> nbr, nbr_cnt;
+    DBuf<int> slot_of[2], nbr_id[2];   // neighbour-search seeds, by step parity
 
     // events (device) + pinned mirror of the counters
     DBuf<int> destroyed_list, replan_list;
@@ -246,6 +247,7 @@ static void free_all(ecm_sim* s) {
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
     s->key.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
+    for (int b = 0; b < 2; ++b) { s->slot_of[b].release(); s->nbr_id[b].release(); }
     s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     for (auto& e : s->envs) e->release();
@@ -287,6 +289,10 @@ static int zero_dev(void* p, size_t bytes, hipStream_t st) {
     HIPCHK(hipMemsetAsync(p, 0, bytes, st));
     return 0;
 }
+static int fill_dev_bytes(void* p, int byte, size_t bytes, hipStream_t st) {
+    HIPCHK(hipMemsetAsync(p, byte, bytes, st));
+    return 0;
+}
 
 extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float x1, float y1, ecm_sim** out) {
     if (!p || !out) return fail(ECM_E_ARG, "null argument");
@@ -324,6 +330,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->sorted_v.alloc(N);
     if (!rc) rc |= s->nbr.alloc(N * (size_t)p->k);
     if (!rc) rc |= s->nbr_cnt.alloc(N);
+    for (int b = 0; b < 2 && !rc; ++b) { rc |= s->slot_of[b].alloc(N); rc |= s->nbr_id[b].alloc(N * (size_t)p->k); }
     if (!rc) rc |= s->destroyed_list.alloc(N);
     if (!rc) rc |= s->replan_list.alloc(N);
     if (!rc) rc |= s->counters.alloc(ecm_sim::kNCounters);
@@ -348,6 +355,10 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc |= zero_dev(s->pref_vel.p, N * sizeof(float2), s->stream);
         rc |= zero_dev(s->active.p, N, s->stream);
         rc |= zero_dev(s->path_cnt.p, N * sizeof(int), s->stream);
+        for (int b = 0; b < 2; ++b) {      // all -1: no seeds before the first step
+            rc |= fill_dev_bytes(s->slot_of[b].p, 0xff, N * sizeof(int), s->stream);
+            rc |= fill_dev_bytes(s->nbr_id[b].p, 0xff, N * (size_t)p->k * sizeof(int), s->stream);
+        }
         rc |= zero_dev(s->path_off.p, N * sizeof(int), s->stream);
         rc |= zero_dev(s->radius.p, N * sizeof(float), s->stream);
         rc |= zero_dev(s->pref_speed.p, N * sizeof(float), s->stream);
@@ -1003,6 +1014,8 @@ static DevState make_state(ecm_sim* s) {
     d.n_keys = s->n_keys;
     d.key = s->key.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
     d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
+    d.slot_of = s->slot_of[s->cur].p; d.slot_of_prev = s->slot_of[s->cur ^ 1].p;
+    d.nbr_id = s->nbr_id[s->cur].p; d.nbr_id_prev = s->nbr_id[s->cur ^ 1].p;
     d.destroyed_list = s->destroyed_list.p; d.destroyed_n = s->counters.p;
     d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
     d.replan_pos = s->replan_pos.p;

@@ -80,6 +80,12 @@ struct DevState {
     float2* sorted_v;       // [n_slots] (vx, vy)
     int* nbr;               // [k][n_slots] neighbour sorted positions (SoA)
     int* nbr_cnt;           // [n_slots]   by sorted position
+    // neighbour-search seeds: the previous step's neighbours of each agent
+    // bound its k-th distance before the search starts (double-buffered)
+    int* slot_of;             // [n_slots] sorted position of each agent, -1 off the grid
+    const int* slot_of_prev;  // the same map of the previous step, null: no seeds
+    int* nbr_id;              // [k][n_slots] neighbour agent ids by sorted position, -1 pad
+    const int* nbr_id_prev;   // the previous step's nbr_id
 
     // ---- events ----
     int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests

@@ -53,7 +53,7 @@ __device__ __forceinline__ int xcd_block() {
 __global__ void k_grid_count(DevState s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
-    if (!s.active[i]) { s.key[i] = kInvalidKey; return; }
+    if (!s.active[i]) { s.key[i] = kInvalidKey; s.slot_of[i] = -1; return; }
     float2 p = s.pos_in[i];
     const EnvDesc& E = env_of(s, i);
     int cx = grid_cell(p.x, E.ag.x0, E.ag.inv_cell, E.ag.side);
@@ -135,6 +135,7 @@ __global__ void k_grid_scatter(DevState s) {
     if (m == kInvalidKey) return;
     uint32_t c = atomicSub(&s.cell_count[m], 1u);
     uint32_t slot = s.cell_start[m] + c - 1u;
+    s.slot_of[i] = (int)slot;
     float2 p = s.pos_in[i];
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
     s.sorted_v[slot] = s.vel_in[i];
@@ -216,31 +217,59 @@ __device__ bool irm_attraction(const EnvDesc& s, V2 pos, const float2* path, int
     float2 g = path[np - 1];
     V2 goal = mk(g.x, g.y);
     if (len2(goal - rl) < clr2) { out = goal; return true; }
+    // The forward loop of the reference keeps the point of the LAST segment
+    // whose circle intersection lies on the segment, and succeeds when any
+    // segment's supporting line meets the circle (disc >= eps, or NaN).
+    // Windows of 32 segments are taken from the end: a cheap pass marks the
+    // segments whose line meets the circle, then the marked ones are solved
+    // from the highest down until one yields a point.  Each segment's
+    // arithmetic is the forward loop's, so the point is the same, and a
+    // lane rarely solves more than one or two segments.
     bool success = false;
-    float2 a = path[0];
-    for (int i = 0; i < np - 1; ++i) {
-        float2 b = path[i + 1];
-        V2 q1 = mk(a.x, a.y) - rl;
-        V2 q2 = mk(b.x, b.y) - rl;
-        a = b;
-        V2 ed = q2 - q1;
-        float el2 = len2(ed);
-        float dm = det(q1, q2);
-        float disc = clr2 * el2 - dm * dm;
-        if (disc < kEps) continue;
-        success = true;
-        float sgn = ed.y < 0.0f ? -1.0f : 1.0f;
-        float sq = sqrtf(disc);
-        V2 i1 = mk((dm * ed.y + sgn * ed.x * sq) / el2, (-dm * ed.x + fabsf(ed.y) * sq) / el2);
-        V2 i2 = mk((dm * ed.y - sgn * ed.x * sq) / el2, (-dm * ed.x - fabsf(ed.y) * sq) / el2);
-        V2 g1 = q1 + rl, g2 = q2 + rl;
-        V2 gi1 = i1 + rl, gi2 = i2 + rl;
-        V2 edg = g2 - g1;
-        float t1 = dot(gi1 - g1, edg) / el2;
-        float t2 = dot(gi2 - g1, edg) / el2;
-        float maxT = -1.0f;
-        if (t1 >= 0.0f && t1 <= 1.0f) { out = gi1; maxT = t1; }
-        if (t2 >= 0.0f && t2 <= 1.0f && t2 > maxT) out = gi2;
+    for (int hi = np - 1; hi > 0; hi -= 32) {
+        const int lo = max(hi - 32, 0);
+        uint32_t mask = 0;
+        float2 a = path[lo];
+#pragma unroll 4
+        for (int i = lo; i < hi; ++i) {
+            const float2 b = path[i + 1];
+            const V2 q1 = mk(a.x, a.y) - rl;
+            const V2 q2 = mk(b.x, b.y) - rl;
+            a = b;
+            const V2 ed = q2 - q1;
+            const float el2 = len2(ed);
+            const float dm = det(q1, q2);
+            const float disc = clr2 * el2 - dm * dm;
+            if (!(disc < kEps)) mask |= 1u << (i - lo);
+        }
+        success |= mask != 0u;
+        while (mask) {
+            const int bit = 31 - __clz(mask);
+            mask &= ~(1u << bit);
+            const int i = lo + bit;
+            const float2 pa = path[i], pb = path[i + 1];
+            const V2 q1 = mk(pa.x, pa.y) - rl;
+            const V2 q2 = mk(pb.x, pb.y) - rl;
+            const V2 ed = q2 - q1;
+            const float el2 = len2(ed);
+            const float dm = det(q1, q2);
+            const float disc = clr2 * el2 - dm * dm;
+            const float sgn = ed.y < 0.0f ? -1.0f : 1.0f;
+            const float sq = sqrtf(disc);
+            const V2 i1 = mk((dm * ed.y + sgn * ed.x * sq) / el2, (-dm * ed.x + fabsf(ed.y) * sq) / el2);
+            const V2 i2 = mk((dm * ed.y - sgn * ed.x * sq) / el2, (-dm * ed.x - fabsf(ed.y) * sq) / el2);
+            const V2 g1 = q1 + rl, g2 = q2 + rl;
+            const V2 gi1 = i1 + rl, gi2 = i2 + rl;
+            const V2 edg = g2 - g1;
+            const float t1 = dot(gi1 - g1, edg) / el2;
+            const float t2 = dot(gi2 - g1, edg) / el2;
+            const bool v1 = t1 >= 0.0f && t1 <= 1.0f;
+            const bool v2 = t2 >= 0.0f && t2 <= 1.0f && t2 > (v1 ? t1 : -1.0f);
+            if (v1 || v2) {
+                out = v2 ? gi2 : gi1;
+                return true;
+            }
+        }
     }
     return success;
 }
@@ -270,38 +299,96 @@ __device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], int (&sl)
     sl[0] = b[0] ? ss : sl[0];
 }
 
-// k-th smallest distance of the list.  With EXACT (k == K) it is the last
-// register; otherwise a select chain, which the compiler may lower to an
-// indexed LDS copy of the list.
-template <int K, bool EXACT>
-__device__ __forceinline__ float nb_kth(const float (&d)[K], int k) {
-    if (EXACT) return d[K - 1];
-    float r = d[0];
+// Upper bound on the squared k-th neighbour distance of agent `id` at p:
+// the largest squared distance to the k agents it had as neighbours in the
+// previous step, when all of them are still on the grid of its environment
+// and none is the agent itself (any k distinct candidates bound the k-th
+// distance from above).  INFINITY when there is no such seed.
+template <int K>   // K == k
+__device__ __forceinline__ float knn_seed_bound(const DevState& s, int id, V2 p) {
+    if (!s.slot_of_prev) return INFINITY;
+    const int q = s.slot_of_prev[id];
+    if (q < 0 || q >= s.n_slots) return INFINITY;
+    const int env = s.agent_env ? (int)s.agent_env[id] : 0;
+    int nid[K];
+    bool ok = true;
 #pragma unroll
-    for (int i = 1; i < K; ++i) r = (i == k - 1) ? d[i] : r;
-    return r;
+    for (int j = 0; j < K; ++j) {
+        nid[j] = s.nbr_id_prev[(size_t)j * s.n_slots + q];
+        ok &= nid[j] >= 0 && nid[j] < s.n_slots;
+    }
+    if (!ok) return INFINITY;
+    int sl[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        sl[j] = s.slot_of[nid[j]];
+        ok &= sl[j] >= 0;
+        if (s.agent_env) ok &= (int)s.agent_env[nid[j]] == env;
+    }
+    if (!ok) return INFINITY;
+    float bound = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const float4 c = s.sorted_pr[sl[j]];
+        const float dx = c.x - p.x, dy = c.y - p.y;
+        const float d2 = dx * dx + dy * dy;
+        ok &= d2 > kEps;
+        bound = fmaxf(bound, d2);
+    }
+    return ok ? bound : INFINITY;
 }
 
 // Exact k nearest by ring search over the agent grid.  Rings are walked as
 // one loop over their 8r perimeter cells; a cell is skipped when its box
 // (open toward the outside on the grid border, where clamped agents live)
-// is farther than the current k-th candidate, and the search stops when the
-// whole unvisited region is.  A 2 mm guard covers rounding.  The list is
-// ordered by (d2, id), so the result does not depend on visiting order.
-template <int K, bool EXACT>
-__device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2 p, int (&out_slot)[K]) {
+// is farther than the current bound on the k-th distance, and the search
+// stops when the whole unvisited region is.  A 2 mm guard covers rounding.
+// Candidates within the bound are appended to a per-lane LDS buffer
+// (`buf`, entry e at buf[e * bstride]) and merged into the ordered register
+// list at the end of each ring or when the buffer is full, so the lanes of
+// a wave rarely wait on each other's list inserts.  The list is ordered by
+// (d2, id), so the result does not depend on visiting order; `bound` (from
+// knn_seed_bound) only prunes.
+template <int K, int B>   // K == k
+__device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2 p, float bound,
+                                         int* buf, int bstride, int (&out_slot)[K], int (&out_id)[K]) {
     const GridDesc& g = E.ag;
     const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
     const int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
     float d[K];
-    int id[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) { d[i] = INFINITY; id[i] = INT_MAX; out_slot[i] = -1; }
-    int n = 0;
+    for (int i = 0; i < K; ++i) { d[i] = INFINITY; out_id[i] = INT_MAX; out_slot[i] = -1; }
+    int n = 0, nb = 0;
     const float guard = 2e-3f;
+    // thr: candidates with d2 <= thr are kept; it is the bound until the
+    // list holds k entries, then the smaller of the bound and the k-th
+    float thr = bound;
     // a cell box at squared distance dd2 is skipped when
-    // sqrt(dd2) - guard > sqrt(kth)  <=>  dd2 > (sqrt(kth) + guard)^2
+    // sqrt(dd2) - guard > sqrt(thr)  <=>  dd2 > (sqrt(thr) + guard)^2
     float prune2 = INFINITY;
+    if (thr < INFINITY) {
+        const float rr = __fsqrt_rn(thr) * 1.0001f + guard;
+        prune2 = rr * rr;
+    }
+    auto merge = [&]() {
+        for (int e = 0; e < nb; ++e) {
+            const int j = buf[e * bstride];
+            const float4 c = s.sorted_pr[j];
+            const float dx = c.x - p.x, dy = c.y - p.y;
+            const float d2 = dx * dx + dy * dy;
+            const int cid = __float_as_int(c.w);
+            if (nb_better(d2, cid, d[K - 1], out_id[K - 1])) {
+                nb_insert<K>(d, out_id, out_slot, d2, cid, j);
+                n = min(n + 1, K);
+            }
+        }
+        nb = 0;
+        if (n >= K) {
+            thr = fminf(thr, d[K - 1]);
+            const float rr = __fsqrt_rn(thr) * 1.0001f + guard;
+            prune2 = rr * rr;
+        }
+    };
     for (int r = 0;; ++r) {
         const int perim = r == 0 ? 1 : 8 * r;
         for (int q = 0; q < perim; ++q) {
@@ -325,38 +412,40 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
             const uint32_t m = E.cell_base + morton2((uint32_t)x, (uint32_t)y);
             const uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
             for (uint32_t j = a; j < b; ++j) {
-                float4 c = s.sorted_pr[j];
-                float dx = c.x - p.x, dy = c.y - p.y;
-                float d2 = dx * dx + dy * dy;
-                int cid = __float_as_int(c.w);
-                if (d2 > kEps && nb_better(d2, cid, d[K - 1], id[K - 1])) {
-                    nb_insert<K>(d, id, out_slot, d2, cid, (int)j);
-                    n = min(n + 1, K);
-                    if (n >= s.k) {
-                        float rr = __fsqrt_rn(nb_kth<K, EXACT>(d, s.k)) * 1.0001f + guard;
-                        prune2 = rr * rr;
-                    }
+                const float2 c = *reinterpret_cast<const float2*>(&s.sorted_pr[j]);
+                const float dx = c.x - p.x, dy = c.y - p.y;
+                const float d2 = dx * dx + dy * dy;
+                if (d2 > kEps && d2 <= thr) {
+                    buf[nb * bstride] = (int)j;
+                    if (++nb == B) merge();
                 }
             }
         }
+        if (nb > 0) merge();
         const int xl = cx - r, xh = cx + r, yl = cy - r, yh = cy + r;
         if (xl <= 0 && yl <= 0 && xh >= g.side - 1 && yh >= g.side - 1) break;
-        if (n >= s.k) {
+        if (thr < INFINITY) {
             float b = INFINITY;
             if (xl > 0) b = fminf(b, p.x - (g.x0 + (float)xl * g.cell));
             if (yl > 0) b = fminf(b, p.y - (g.y0 + (float)yl * g.cell));
             if (xh < g.side - 1) b = fminf(b, (g.x0 + (float)(xh + 1) * g.cell) - p.x);
             if (yh < g.side - 1) b = fminf(b, (g.y0 + (float)(yh + 1) * g.cell) - p.y);
             b -= guard;
-            if (b > 0.0f && b * b > nb_kth<K, EXACT>(d, s.k)) break;
+            if (b > 0.0f && b * b > thr) break;
         }
     }
-    return min(n, s.k);
+    return n;
 }
 
 // --------------------------------------- attraction + steering + neighbours
-template <int K, bool EXACT>
+// Candidate buffer of the neighbour search: kKnnBuf sorted positions per
+// lane in dynamic LDS (kKnnBuf * blockDim.x ints per block).
+static constexpr int kKnnBuf = 16;
+
+// K == k: the k-th neighbour is the last register of the list.
+template <int K>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
+    extern __shared__ int knn_lds[];
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
@@ -395,6 +484,7 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
         uint32_t q = atomicAdd(s.destroyed_n, 1u);
         s.destroyed_list[q] = id;
         s.nbr_cnt[t] = 0;
+        for (int i = 0; i < K; ++i) s.nbr_id[(size_t)i * s.n_slots + t] = -1;
         return;
     }
     // ApplySteeringForce (Simulator.cpp:638-657)
@@ -403,11 +493,14 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     s.pref_vel[id] = make_float2(des.x, des.y);
 
     // k nearest (replaces KDTree::KNearestAgents)
-    int slot[K];
-    const int cnt = knn_query<K, EXACT>(s, E, pos, slot);
+    int slot[K], nid[K];
+    const float bound = knn_seed_bound<K>(s, id, pos);
+    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, slot, nid);
 #pragma unroll
-    for (int i = 0; i < K; ++i)
+    for (int i = 0; i < K; ++i) {
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
+        s.nbr_id[(size_t)i * s.n_slots + t] = i < cnt ? nid[i] : -1;
+    }
     s.nbr_cnt[t] = cnt;
 }
 
@@ -1133,12 +1226,15 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, hip
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
     if (block != 64 && block != 128) block = 256;
     dim3 g(round8(cdiv(s.n_slots, block))), b(block);
-    if (s.k == 5) hipLaunchKernelGGL((k_attr_knn<5, true>), g, b, 0, st, s);
-    else if (s.k == 10) hipLaunchKernelGGL((k_attr_knn<10, true>), g, b, 0, st, s);
-    else if (s.k < 5) hipLaunchKernelGGL((k_attr_knn<5, false>), g, b, 0, st, s);
-    else if (s.k < 10) hipLaunchKernelGGL((k_attr_knn<10, false>), g, b, 0, st, s);
-    else if (s.k == 16) hipLaunchKernelGGL((k_attr_knn<16, true>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_attr_knn<16, false>), g, b, 0, st, s);
+    const size_t lds = (size_t)kKnnBuf * block * sizeof(int);
+    switch (s.k) {
+#define ECM_KNN_CASE(k) case k: hipLaunchKernelGGL((k_attr_knn<k>), g, b, lds, st, s); break;
+        ECM_KNN_CASE(1) ECM_KNN_CASE(2) ECM_KNN_CASE(3) ECM_KNN_CASE(4) ECM_KNN_CASE(5) ECM_KNN_CASE(6)
+        ECM_KNN_CASE(7) ECM_KNN_CASE(8) ECM_KNN_CASE(9) ECM_KNN_CASE(10) ECM_KNN_CASE(11) ECM_KNN_CASE(12)
+        ECM_KNN_CASE(13) ECM_KNN_CASE(14) ECM_KNN_CASE(15) ECM_KNN_CASE(16)
+#undef ECM_KNN_CASE
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

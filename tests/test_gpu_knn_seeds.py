@@ -1,0 +1,111 @@
+"""The neighbour search starts from a bound taken from each agent's previous
+neighbours.  Those seeds go stale when agents are destroyed, when their
+slots are reused by new agents (possibly in another environment) and when
+a snapshot from another time is loaded; the step must still equal the
+oracle bit for bit (the bound only prunes, and a seed that is not a
+candidate any more disables it)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from ecmgenerator_amd.simulator import Agents, Params, Simulator
+from helpers import compare_step, oracle_state, small_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def subset(agents: Agents, idx) -> Agents:
+    idx = np.asarray(idx)
+    cnt = agents.path_cnt[idx]
+    off = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(agents.path_off.dtype)
+    xy = np.concatenate([agents.path_xy[o:o + c] for o, c in zip(agents.path_off[idx], cnt)])
+    return Agents(agents.start[idx], agents.radius[idx], agents.pref_speed[idx], off,
+                  cnt.astype(agents.path_cnt.dtype), xy)
+
+
+def check_step(sim, p, obs, ecm, tag):
+    from oracle import oracle_py as O
+    before = oracle_state(sim)
+    ora = O.step(p, before, obs, ecm)
+    sim.Update()
+    res = compare_step(sim, before, ora)
+    print(f"{tag}: {res}", flush=True)
+    assert all(v == 0 for v in res.values()), (tag, res)
+
+
+def test_reused_slots_and_destroyed_neighbours():
+    box, _, obs, ecm, agents = small_scene(41, half=30.0, n_rects=4, n_agents=1800)
+    p = Params(max_agents=2048, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    first = subset(agents, np.arange(0, 1400))
+    ids = sim.SpawnAgents(first)
+    for _ in range(6):
+        sim.Update()
+    # destroy every third live agent: their neighbours lose seeds, and the
+    # freed slots are handed to new agents elsewhere (LIFO free list)
+    live = np.flatnonzero(sim.state()["active"].astype(bool))
+    for i in live[::3]:
+        sim.DestroyAgent(int(i))
+    check_step(sim, p, obs, ecm, "after destroy")
+    new = sim.SpawnAgents(subset(agents, np.arange(1400, 1800)))
+    reused = np.intersect1d(new[new >= 0], ids[ids >= 0])
+    print(f"spawned {int((new >= 0).sum())}, {reused.size} into reused slots", flush=True)
+    assert reused.size > 100
+    for it in range(3):
+        check_step(sim, p, obs, ecm, f"after respawn step {it}")
+    sim.close()
+
+
+def test_snapshot_from_another_time():
+    box, _, obs, ecm, agents = small_scene(43, half=30.0, n_rects=4, n_agents=1500)
+    p = Params(max_agents=2048, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    sim.SpawnAgents(agents)
+    for _ in range(3):
+        sim.Update()
+    with tempfile.TemporaryDirectory() as d:
+        snap = os.path.join(d, "s.bin")
+        sim.snapshot_save(snap)
+        for _ in range(25):          # seeds now describe a much later crowd
+            sim.Update()
+        sim.snapshot_load(snap)
+        for it in range(2):
+            check_step(sim, p, obs, ecm, f"after load step {it}")
+    sim.close()
+
+
+def test_slots_reused_across_environments():
+    """Agents of environment 0 are destroyed and their slots reused by
+    environment 1, whose world overlaps the same coordinates: the old
+    neighbour lists point into the other environment and must be ignored."""
+    from oracle import oracle_py as O
+    scenes = [small_scene(51, half=30.0, n_rects=4, n_agents=900),
+              small_scene(52, half=30.0, n_rects=4, n_agents=900)]
+    p = Params(max_agents=2048, keep_orca_velocity=True)
+    box, _, obs, ecm, agents = scenes[0]
+    sim = Simulator(ecm, obs, box, p)
+    ids0 = sim.SpawnAgents(agents)
+    e1 = sim.AddEnvironment(scenes[1][3], scenes[1][2], scenes[1][0])
+    for _ in range(5):
+        sim.Update()
+    live = np.flatnonzero(sim.state()["active"].astype(bool))
+    for i in live[::2]:
+        sim.DestroyAgent(int(i))
+    ids1 = sim.SpawnAgents(scenes[1][4], env=e1)
+    assert np.intersect1d(ids1[ids1 >= 0], ids0[ids0 >= 0]).size > 100
+    for it in range(3):
+        before = oracle_state(sim)
+        env = sim.agent_environments()
+        oras = []
+        for e, (box_e, _, obs_e, ecm_e, _) in enumerate(scenes):
+            st = dict(before)
+            st["active"] = (before["active"].astype(bool) & (env == e)).astype(np.uint8)
+            oras.append((st, O.step(p, st, obs_e, ecm_e)))
+        sim.Update()
+        for e, (st, ora) in enumerate(oras):
+            res = compare_step(sim, st, ora, subset=np.flatnonzero(env == e))
+            print(f"step {it} env {e}: {res}", flush=True)
+            assert all(v == 0 for v in res.values()), (it, e, res)
+    sim.close()
