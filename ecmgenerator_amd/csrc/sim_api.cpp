@@ -131,7 +131,7 @@ struct ecm_sim {
     int cur = 0;
 
     // agent grid
-    DBuf<uint32_t> key, cell_count, cell_start, bsum;
+    DBuf<uint32_t> key, rank, cell_count, cell_start, bsum;
     DBuf<float4> sorted_pr;
     DBuf<float2> sorted_v;
     DBuf<int> nbr, nbr_cnt;
@@ -245,7 +245,7 @@ static void free_all(ecm_sim* s) {
     s->attraction.release(); s->pref_vel.release(); s->v_orca.release();
     s->radius.release(); s->pref_speed.release(); s->active.release();
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
-    s->key.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
+    s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
     for (int b = 0; b < 2; ++b) { s->slot_of[b].release(); s->nbr_id[b].release(); }
     s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
@@ -326,6 +326,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->path_cnt.alloc(N);
     if (!rc) rc |= s->path_pool.alloc(std::max<size_t>(N * 8, 1024));
     if (!rc) rc |= s->key.alloc(N);
+    if (!rc) rc |= s->rank.alloc(N);
     if (!rc) rc |= s->sorted_pr.alloc(N);
     if (!rc) rc |= s->sorted_v.alloc(N);
     if (!rc) rc |= s->nbr.alloc(N * (size_t)p->k);
@@ -1012,7 +1013,7 @@ static DevState make_state(ecm_sim* s) {
     d.envs = s->d_envs.p;
     d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
     d.n_keys = s->n_keys;
-    d.key = s->key.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
+    d.key = s->key.p; d.rank = s->rank.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
     d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
     d.slot_of = s->slot_of[s->cur].p; d.slot_of_prev = s->slot_of[s->cur ^ 1].p;
     d.nbr_id = s->nbr_id[s->cur].p; d.nbr_id_prev = s->nbr_id[s->cur ^ 1].p;

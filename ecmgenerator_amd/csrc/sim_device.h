@@ -74,7 +74,8 @@ struct DevState {
     // ---- agent grid (rebuilt every step) ----
     uint32_t n_keys;        // cells of all environments
     uint32_t* key;          // [n_slots] cell key or 0xffffffff
-    uint32_t* cell_count;   // [n_keys]   zero between steps
+    uint32_t* rank;         // [n_slots] arrival order of the agent in its cell
+    uint32_t* cell_count;   // [n_keys]   zero between steps (the scan clears it)
     uint32_t* cell_start;   // [n_keys + 1] exclusive scan; [n_keys] = active count
     float4* sorted_pr;      // [n_slots] (x, y, radius, id bits)
     float2* sorted_v;       // [n_slots] (vx, vy)

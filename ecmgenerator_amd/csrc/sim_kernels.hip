@@ -60,7 +60,7 @@ __global__ void k_grid_count(DevState s) {
     int cy = grid_cell(p.y, E.ag.y0, E.ag.inv_cell, E.ag.side);
     uint32_t m = E.cell_base + morton2((uint32_t)cx, (uint32_t)cy);
     s.key[i] = m;
-    atomicAdd(&s.cell_count[m], 1u);
+    s.rank[i] = atomicAdd(&s.cell_count[m], 1u);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -85,12 +85,16 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t& tot
     __syncthreads();
     return off + inc - v;
 }
-// 1024 counts per block -> local exclusive offsets + block sums
-__global__ __launch_bounds__(256) void k_scan_local(const uint32_t* cnt, uint32_t* start, uint32_t* bsum, uint32_t n) {
+// 1024 counts per block -> local exclusive offsets + block sums; the
+// counts are cleared for the next step
+__global__ __launch_bounds__(256) void k_scan_local(uint32_t* cnt, uint32_t* start, uint32_t* bsum, uint32_t n) {
     uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
     uint32_t a[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) a[q] = (base + q < n) ? cnt[base + q] : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (base + q < n) cnt[base + q] = 0u;
     uint32_t loc = a[0] + a[1] + a[2] + a[3];
     uint32_t tot;
     uint32_t ex = block_excl_scan256(loc, tot);
@@ -126,15 +130,14 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* start, const uint32_
     if (i < n) start[i] += bsum[i >> 10];
 }
 
-// Counting-sort scatter.  Decrementing the counts leaves cell_count zeroed
-// for the next step.
+// Counting-sort scatter: the slot is the cell's start plus the arrival
+// order the count pass recorded.
 __global__ void k_grid_scatter(DevState s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
     uint32_t m = s.key[i];
     if (m == kInvalidKey) return;
-    uint32_t c = atomicSub(&s.cell_count[m], 1u);
-    uint32_t slot = s.cell_start[m] + c - 1u;
+    uint32_t slot = s.cell_start[m] + s.rank[i];
     s.slot_of[i] = (int)slot;
     float2 p = s.pos_in[i];
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
@@ -1188,7 +1191,7 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
     const uint32_t G = s.n_keys;
     hipLaunchKernelGGL(k_grid_count, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
     const uint32_t nb = (G + 1023u) / 1024u;
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, st, (const uint32_t*)s.cell_count, s.cell_start, bsum, G);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, st, s.cell_count, s.cell_start, bsum, G);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb, s.cell_start, G);
     hipLaunchKernelGGL(k_scan_add, dim3(cdiv(G, 256)), dim3(256), 0, st, s.cell_start, (const uint32_t*)bsum, G);
     hipLaunchKernelGGL(k_grid_scatter, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
