@@ -283,11 +283,11 @@ __device__ __forceinline__ bool nb_better(float d, int id, float d2, int id2) {
     return (d < d2) | ((d == d2) & (id < id2));      // bitwise: no branches
 }
 
-// Insert (dd, ii, ss) into the ascending register list (d, id, slot) of
-// length K: one comparison per entry (the flags are monotone along the
-// list), then a predicated shift; no dynamic indexing.
+// Insert (dd, ii) into the ascending register list (d, id) of length K:
+// one comparison per entry (the flags are monotone along the list), then a
+// predicated shift; no dynamic indexing.
 template <int K>
-__device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], int (&sl)[K], float dd, int ii, int ss) {
+__device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], float dd, int ii) {
     bool b[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) b[i] = nb_better(dd, ii, d[i], id[i]);
@@ -295,11 +295,9 @@ __device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], int (&sl)
     for (int i = K - 1; i > 0; --i) {
         d[i] = b[i - 1] ? d[i - 1] : (b[i] ? dd : d[i]);
         id[i] = b[i - 1] ? id[i - 1] : (b[i] ? ii : id[i]);
-        sl[i] = b[i - 1] ? sl[i - 1] : (b[i] ? ss : sl[i]);
     }
     d[0] = b[0] ? dd : d[0];
     id[0] = b[0] ? ii : id[0];
-    sl[0] = b[0] ? ss : sl[0];
 }
 
 // Upper bound on the squared k-th neighbour distance of agent `id` at p:
@@ -351,16 +349,17 @@ __device__ __forceinline__ float knn_seed_bound(const DevState& s, int id, V2 p)
 // list at the end of each ring or when the buffer is full, so the lanes of
 // a wave rarely wait on each other's list inserts.  The list is ordered by
 // (d2, id), so the result does not depend on visiting order; `bound` (from
-// knn_seed_bound) only prunes.
+// knn_seed_bound) only prunes.  The list holds agent ids; their sorted
+// positions come from slot_of afterwards.
 template <int K, int B>   // K == k
 __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2 p, float bound,
-                                         int* buf, int bstride, int (&out_slot)[K], int (&out_id)[K]) {
+                                         int* buf, int bstride, int (&out_id)[K]) {
     const GridDesc& g = E.ag;
     const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
     const int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
     float d[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) { d[i] = INFINITY; out_id[i] = INT_MAX; out_slot[i] = -1; }
+    for (int i = 0; i < K; ++i) { d[i] = INFINITY; out_id[i] = INT_MAX; }
     int n = 0, nb = 0;
     const float guard = 2e-3f;
     // thr: candidates with d2 <= thr are kept; it is the bound until the
@@ -381,7 +380,7 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
             const float d2 = dx * dx + dy * dy;
             const int cid = __float_as_int(c.w);
             if (nb_better(d2, cid, d[K - 1], out_id[K - 1])) {
-                nb_insert<K>(d, out_id, out_slot, d2, cid, j);
+                nb_insert<K>(d, out_id, d2, cid);
                 n = min(n + 1, K);
             }
         }
@@ -498,7 +497,12 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K], nid[K];
     const float bound = knn_seed_bound<K>(s, id, pos);
-    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, slot, nid);
+    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, nid);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const int sl = s.slot_of[i < cnt ? nid[i] : 0];   // unused entries read slot 0
+        slot[i] = i < cnt ? sl : -1;
+    }
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
