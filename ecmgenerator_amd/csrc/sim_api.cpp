@@ -137,16 +137,27 @@ struct ecm_sim {
     DBuf<int> nbr, nbr_cnt;
     DBuf<int> slot_of[2], nbr_id[2];   // neighbour-search seeds, by step parity
 
-    // events (device) + pinned mirror of the counters
-    DBuf<int> destroyed_list, replan_list;
-    DBuf<float2> replan_pos;
-    static const int kReplanPeek = 1024;     // replan requests read back while k_orca runs
-    int* h_replan_ids = nullptr;             // pinned [kReplanPeek]
-    float2* h_replan_pos = nullptr;          // pinned [kReplanPeek]
-    // [0] destroyed_n [1] replan_n [2..7] stats [8..263] per-wavefront slots
-    static const int kNCounters = 8 + 256;
-    DBuf<uint32_t> counters;
-    uint32_t* h_counters = nullptr;
+    // Per-step counters and the event list share one device block, so one
+    // copy brings both back: words [0..7] are the counters ([0] events,
+    // [2..4] stats), then one int4 per event.  The per-wavefront slots and
+    // the 64-bit totals of finished steps live apart.
+    static constexpr int kEventPeek = 1024;      // events read back while k_orca runs
+    DBuf<uint32_t> xfer;                     // 8 + 4 * max_agents words
+    DBuf<uint32_t> wstats;                   // [64][4]
+    DBuf<unsigned long long> acc;            // [16] totals (folded in by k_grid_count); [12] steps run
+    // host-mapped, 1 + 8 + 4 * kEventPeek words: [0] the number of the last
+    // step that published, then its counters and first events (k_publish)
+    uint32_t* h_xfer = nullptr;
+    uint32_t* h_xfer_dev = nullptr;          // the same memory as the kernels see it
+    uint32_t step_seq = 0;                   // steps launched (matches acc[12])
+    // Replan upload staging, reused step after step; ev_stage marks the end
+    // of the last upload that read it.
+    int* h_stage = nullptr;                  // pinned ids, offsets, counts
+    size_t h_stage_cap = 0;
+    float2* h_pts = nullptr;                 // pinned path points
+    size_t h_pts_cap = 0;
+    DBuf<int> d_stage;
+    hipEvent_t ev_stage = nullptr;
 
     // environments (0 is created with the handle)
     std::vector<std::unique_ptr<EnvHost>> envs;
@@ -181,7 +192,6 @@ struct ecm_sim {
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t ev_replan = nullptr;
     float last_ms[4] = {0, 0, 0, 0};
 
     // learned proposal (features + network + verification) and sample log
@@ -248,18 +258,18 @@ static void free_all(ecm_sim* s) {
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
     for (int b = 0; b < 2; ++b) { s->slot_of[b].release(); s->nbr_id[b].release(); }
-    s->destroyed_list.release(); s->replan_list.release(); s->counters.release();
+    s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     for (auto& e : s->envs) e->release();
     s->d_envs.release();
     s->agent_env.release();
-    if (s->h_counters) (void)hipHostFree(s->h_counters);
-    s->h_counters = nullptr;
-    if (s->h_replan_ids) (void)hipHostFree(s->h_replan_ids);
-    if (s->h_replan_pos) (void)hipHostFree(s->h_replan_pos);
-    s->h_replan_ids = nullptr;
-    s->h_replan_pos = nullptr;
-    s->replan_pos.release();
+    if (s->h_xfer) (void)hipHostFree(s->h_xfer);
+    s->h_xfer = nullptr;
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    if (s->h_pts) (void)hipHostFree(s->h_pts);
+    s->h_stage = nullptr;
+    s->h_pts = nullptr;
+    s->h_stage_cap = s->h_pts_cap = 0;
     s->feat.release();
     s->mlp_mask.release();
     for (auto& g : s->graphs)
@@ -271,8 +281,8 @@ static void free_all(ecm_sim* s) {
             g.valid = false;
         }
     for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
-    if (s->ev_replan) (void)hipEventDestroy(s->ev_replan);
-    s->ev_replan = nullptr;
+    if (s->ev_stage) (void)hipEventDestroy(s->ev_stage);
+    s->ev_stage = nullptr;
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -283,6 +293,20 @@ extern "C" void ecm_sim_destroy(ecm_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     free_all(s);
     delete s;
+}
+
+static float bits_to_float(int v) {
+    float f;
+    std::memcpy(&f, &v, sizeof(f));
+    return f;
+}
+
+// Waits for the work still queued on the handle's stream.  A step returns
+// once its host part is done while its last kernels may still run, so
+// every entry point that reads or replaces device data settles first.
+static int settle(ecm_sim* s) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
 static int zero_dev(void* p, size_t bytes, hipStream_t st) {
@@ -332,21 +356,20 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->nbr.alloc(N * (size_t)p->k);
     if (!rc) rc |= s->nbr_cnt.alloc(N);
     for (int b = 0; b < 2 && !rc; ++b) { rc |= s->slot_of[b].alloc(N); rc |= s->nbr_id[b].alloc(N * (size_t)p->k); }
-    if (!rc) rc |= s->destroyed_list.alloc(N);
-    if (!rc) rc |= s->replan_list.alloc(N);
-    if (!rc) rc |= s->counters.alloc(ecm_sim::kNCounters);
-    if (!rc) rc |= s->replan_pos.alloc(N);
+    if (!rc) rc |= s->xfer.alloc(8 + 4 * N);
+    if (!rc) rc |= s->wstats.alloc(256);
+    if (!rc) rc |= s->acc.alloc(16);
     if (!rc) rc |= s->agent_env.alloc(N);
-    if (!rc && hipHostMalloc((void**)&s->h_counters, ecm_sim::kNCounters * sizeof(uint32_t)) != hipSuccess)
+    if (!rc && hipHostMalloc((void**)&s->h_xfer, (1 + 8 + 4 * (size_t)ecm_sim::kEventPeek) * sizeof(uint32_t),
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
-    if (!rc && hipHostMalloc((void**)&s->h_replan_ids, ecm_sim::kReplanPeek * sizeof(int)) != hipSuccess)
-        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
-    if (!rc && hipHostMalloc((void**)&s->h_replan_pos, ecm_sim::kReplanPeek * sizeof(float2)) != hipSuccess)
-        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
+    if (!rc && hipHostGetDevicePointer((void**)&s->h_xfer_dev, s->h_xfer, 0) != hipSuccess)
+        rc = fail(ECM_E_HIP, "hipHostGetDevicePointer failed");
+    if (!rc) s->h_xfer[0] = 0u;
+    if (!rc && hipEventCreateWithFlags(&s->ev_stage, hipEventDisableTiming) != hipSuccess)
+        rc = fail(ECM_E_HIP, "event creation failed");
     for (int i = 0; i < 5 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
-    if (!rc && hipEventCreateWithFlags(&s->ev_replan, hipEventDisableTiming) != hipSuccess)
-        rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc) {
         for (int b = 0; b < 2 && !rc; ++b) {
             rc |= zero_dev(s->pos[b].p, N * sizeof(float2), s->stream);
@@ -363,7 +386,9 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc |= zero_dev(s->path_off.p, N * sizeof(int), s->stream);
         rc |= zero_dev(s->radius.p, N * sizeof(float), s->stream);
         rc |= zero_dev(s->pref_speed.p, N * sizeof(float), s->stream);
-        rc |= zero_dev(s->counters.p, 8 * sizeof(uint32_t), s->stream);
+        rc |= zero_dev(s->xfer.p, 8 * sizeof(uint32_t), s->stream);
+        rc |= zero_dev(s->wstats.p, 256 * sizeof(uint32_t), s->stream);
+        rc |= zero_dev(s->acc.p, 16 * sizeof(unsigned long long), s->stream);
         if (s->v_orca.p) rc |= zero_dev(s->v_orca.p, N * sizeof(float2), s->stream);
         if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(ECM_E_HIP, "sync failed");
     }
@@ -490,6 +515,7 @@ extern "C" int ecm_sim_env_set_obstacles(ecm_sim* s, int env, const float* xy, c
     if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
     EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     std::vector<int> ro(ring_off, ring_off + n_rings + 1);
     if (n_rings == 0) ro.assign(1, 0);
     if (ro[0] != 0) return fail(ECM_E_ARG, "ring_off[0] must be 0");
@@ -562,6 +588,7 @@ extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float*
     if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
     EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     for (int e = 0; e < 2 * n_edge; ++e)
         if (he_target[e] < 0 || he_target[e] >= n_vert) return fail(ECM_E_ARG, "half-edge target out of range");
     E->ecm_vert.assign(vert_xy, vert_xy + 2 * (size_t)n_vert);
@@ -725,6 +752,7 @@ extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_
     if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
     EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     if (n == 0) return 0;
     for (int i = 0; i < n; ++i)
         if (path_cnt[i] < 1 || path_off[i] < 0)
@@ -847,6 +875,7 @@ extern "C" int ecm_sim_destroy_agent(ecm_sim* s, int id) {
     if (!s || id < 0 || id >= s->prm.max_agents) return fail(ECM_E_ARG, "bad agent id");
     if (!s->h_active[id]) return fail(ECM_E_ARG, "agent %d is not active", id);
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     uint8_t z = 0;
     HIPCHK(hipMemcpyAsync(s->active.p + id, &z, 1, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -867,6 +896,7 @@ extern "C" int ecm_sim_reset(ecm_sim* s) {
 extern "C" int ecm_sim_set_path(ecm_sim* s, int id, const float* xy, int n) {
     if (!s || id < 0 || id >= s->prm.max_agents || n < 1 || !xy) return fail(ECM_E_ARG, "bad set_path arguments");
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     int rc = ensure_path_capacity(s, (size_t)n);
     if (rc) return rc;
     int off = (int)s->path_used;
@@ -895,6 +925,8 @@ extern "C" int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* u
 // ---------------------------------------------------------- environments
 extern "C" int ecm_sim_add_env(ecm_sim* s, float x0, float y0, float x1, float y1, int* env) {
     if (!s || !env) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     if (!(x1 > x0) || !(y1 > y0)) return fail(ECM_E_ARG, "empty world box");
     if (s->envs.size() >= 65535) return fail(ECM_E_FULL, "too many environments");
     std::unique_ptr<EnvHost> e(new EnvHost());
@@ -937,6 +969,8 @@ static int ensure_agent_grid(ecm_sim* s) {
         else if (s->prm.grid_cell <= 0.0f && (n > 2 * e->grid_for_n || 2 * n < e->grid_for_n)) rebuild = true;
     }
     if (!rebuild) return 0;
+    int rc = settle(s);            // the previous step may still read the grid
+    if (rc) return rc;
     uint64_t base = 0;
     for (auto& e : s->envs) {
         const int n = std::max(1, e->num_agents);
@@ -953,7 +987,7 @@ static int ensure_agent_grid(ecm_sim* s) {
     }
     if (base >= 0xffffffffull) return fail(ECM_E_STATE, "agent grids of the environments exceed 2^32 cells");
     const size_t G = (size_t)base;
-    int rc = s->cell_count.alloc(G);
+    rc = s->cell_count.alloc(G);
     if (!rc) rc = s->cell_start.alloc(G + 1);
     if (!rc) rc = s->bsum.alloc((G + 1023) / 1024 + 1);
     if (rc) return rc;
@@ -968,6 +1002,10 @@ static int ensure_agent_grid(ecm_sim* s) {
 // Device copy of the environment descriptors and of the slot -> environment
 // map, refreshed when the host side changed.
 static int upload_envs(ecm_sim* s) {
+    if (s->envs_dirty || s->agent_env_dirty) {
+        int rc = settle(s);        // the previous step may still read them
+        if (rc) return rc;
+    }
     if (s->envs_dirty) {
         std::vector<EnvDesc> h(s->envs.size());
         for (size_t i = 0; i < s->envs.size(); ++i) {
@@ -1017,11 +1055,12 @@ static DevState make_state(ecm_sim* s) {
     d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
     d.slot_of = s->slot_of[s->cur].p; d.slot_of_prev = s->slot_of[s->cur ^ 1].p;
     d.nbr_id = s->nbr_id[s->cur].p; d.nbr_id_prev = s->nbr_id[s->cur ^ 1].p;
-    d.destroyed_list = s->destroyed_list.p; d.destroyed_n = s->counters.p;
-    d.replan_list = s->replan_list.p; d.replan_n = s->counters.p + 1;
-    d.replan_pos = s->replan_pos.p;
-    d.stats = s->counters.p + 2;
-    d.wstats = s->counters.p + 8;
+    d.events = reinterpret_cast<int4*>(s->xfer.p + 8); d.events_n = s->xfer.p;
+    d.stats = s->xfer.p + 2;
+    d.wstats = s->wstats.p;
+    d.acc = s->acc.p;
+    d.host_xfer = s->h_xfer_dev;
+    d.event_peek = (uint32_t)std::min(ecm_sim::kEventPeek, s->prm.max_agents);
     d.ovf_agent = s->ovf_agent.p;
     d.feat = s->feat.p;
     d.mlp_mask = s->mlp_mask.p;
@@ -1095,17 +1134,43 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     if (ids.empty()) return 0;
     int rc = ensure_path_capacity(s, pts.size());
     if (rc) return rc;
-    DBuf<int> d;
-    rc = d.alloc(3 * ids.size());
-    if (rc) return rc;
     const size_t m = ids.size();
-    HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, pts.data(), pts.size() * sizeof(float2), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(d.p, ids.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(d.p + m, offs.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(d.p + 2 * m, cnts.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(launch_path_scatter((int)m, d.p, d.p + m, d.p + 2 * m, s->path_off.p, s->path_cnt.p, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    d.release();
+    // The pinned staging is reused from step to step without a stream
+    // sync: wait for the copies of the last upload that read it (long done
+    // in the usual case), and grow it only when the stream is idle.
+    HIPCHK(hipEventSynchronize(s->ev_stage));
+    if (s->h_stage_cap < 3 * m || s->h_pts_cap < pts.size() || s->d_stage.n < 3 * m) {
+        if ((rc = settle(s)) != 0) return rc;
+        if (s->h_stage_cap < 3 * m) {
+            if (s->h_stage) (void)hipHostFree(s->h_stage);
+            s->h_stage = nullptr;
+            s->h_stage_cap = std::max<size_t>(3 * m, 3 * 1024);
+            if (hipHostMalloc((void**)&s->h_stage, s->h_stage_cap * sizeof(int)) != hipSuccess) {
+                s->h_stage_cap = 0;
+                return fail(ECM_E_HIP, "hipHostMalloc failed");
+            }
+        }
+        if (s->h_pts_cap < pts.size()) {
+            if (s->h_pts) (void)hipHostFree(s->h_pts);
+            s->h_pts = nullptr;
+            s->h_pts_cap = std::max<size_t>(pts.size(), 32 * 1024);
+            if (hipHostMalloc((void**)&s->h_pts, s->h_pts_cap * sizeof(float2)) != hipSuccess) {
+                s->h_pts_cap = 0;
+                return fail(ECM_E_HIP, "hipHostMalloc failed");
+            }
+        }
+        if (s->d_stage.n < 3 * m && (rc = s->d_stage.alloc(std::max<size_t>(3 * m, 3 * 1024))) != 0) return rc;
+    }
+    std::memcpy(s->h_stage, ids.data(), m * sizeof(int));
+    std::memcpy(s->h_stage + m, offs.data(), m * sizeof(int));
+    std::memcpy(s->h_stage + 2 * m, cnts.data(), m * sizeof(int));
+    std::memcpy(s->h_pts, pts.data(), pts.size() * sizeof(float2));
+    HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, s->h_pts, pts.size() * sizeof(float2), hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_stage.p, s->h_stage, 3 * m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipEventRecord(s->ev_stage, s->stream));
+    HIPCHK(launch_path_scatter((int)m, s->d_stage.p, s->d_stage.p + m, s->d_stage.p + 2 * m, s->path_off.p,
+                               s->path_cnt.p, s->stream));
     s->path_used += pts.size();
     return 0;
 }
@@ -1122,6 +1187,7 @@ static int prepare_step(ecm_sim* s, DevState* out) {
         int maxo = 16;
         for (auto& e : s->envs) {
             if (e->n_obst_edges > 0 && e->obst_range_built < range) {
+                if ((rc = settle(s)) != 0) return rc;
                 rc = build_obstacle_buckets(s, e.get(), range * 1.25f + 0.5f);
                 if (rc) return rc;
             }
@@ -1134,6 +1200,7 @@ static int prepare_step(ecm_sim* s, DevState* out) {
     {
         int stride = s->maxo + s->prm.k;
         if (stride != s->q3d_stride) {
+            if ((rc = settle(s)) != 0) return rc;
             const size_t N = (size_t)s->prm.max_agents;
             rc = s->q3d_agent.alloc(N);
             if (!rc) rc = s->ovf_agent.alloc(N);
@@ -1155,18 +1222,15 @@ static int step_mode(const ecm_sim* s) { return s->log_mlp ? 1 : (s->mlp ? 2 : 0
 // host synchronization: each half is what one step graph captures.
 static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half) {
     if (half == 0) {
-        // destroyed / replan lists and every per-step counter restart
-        HIPCHK(launch_fill_u32(s->counters.p, 0u, ecm_sim::kNCounters, s->stream));
         if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
+        // block 0 of the count pass also folds the previous step's counters
+        // into the totals and clears them for this step
         HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
         if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
         HIPCHK(launch_attr_knn(d, s->tune.knn_block, s->stream));
-        // replan requests are final once k_attr_knn is done: read them back
-        // and plan on the host while k_orca runs
-        const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
-        HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipMemcpyAsync(s->h_replan_ids, s->replan_list.p, peek * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipMemcpyAsync(s->h_replan_pos, s->replan_pos.p, peek * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+        // replan and destroy events are final once k_attr_knn is done;
+        // k_publish (launched right after it) writes them to host memory,
+        // and the host handles them while k_orca runs
         return 0;
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
@@ -1179,8 +1243,28 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
     HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
-    HIPCHK(hipMemcpyAsync(s->h_counters, s->counters.p, ecm_sim::kNCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          s->stream));
+    return 0;
+}
+
+// Waits until step `seq` has published its events (k_publish writes the
+// step number last, after a system-scope fence).  The stream is polled
+// now and then so that a step that ends without publishing is an error,
+// not a hang.
+static int wait_published(ecm_sim* s, uint32_t seq) {
+    const volatile uint32_t* flag = s->h_xfer;
+    for (uint64_t spin = 1; *flag != seq; ++spin) {
+        if ((spin & 255u) == 0) {
+            const hipError_t q = hipStreamQuery(s->stream);
+            if (q == hipSuccess) {
+                if (*flag == seq) break;
+                s->step_seq = *flag;           // resynchronise with what the device counted
+                return fail(ECM_E_STATE, "step %u finished without publishing its events", seq);
+            }
+            if (q != hipErrorNotReady) return fail(ECM_E_HIP, "hipStreamQuery: %s", hipGetErrorString(q));
+            std::this_thread::yield();
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     return 0;
 }
 
@@ -1211,15 +1295,17 @@ static int ensure_step_graph(ecm_sim* s, const DevState& d, int mode) {
     key.n_cu = mode == 2 ? s->mlp->n_cu : 0;
     ecm_sim::StepGraph& g = s->graphs[s->cur];
     if (g.valid && std::memcmp(&g.key, &key, sizeof(key)) == 0) return 0;
+    if (int rc = settle(s)) return rc;      // an earlier launch of g may still run
     drop_graph(g);
-    for (int h = 0; h < 2; ++h) {
+    {
         hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) {
             s->graph_error = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
             drop_graph(g);
             return 1;
         }
-        int rc = enqueue_device_half(s, d, mode, h);
+        int rc = enqueue_device_half(s, d, mode, 0);
+        if (!rc) rc = enqueue_device_half(s, d, mode, 1);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(s->stream, &graph);
         hipGraphExec_t exec = nullptr;
@@ -1232,8 +1318,8 @@ static int ensure_step_graph(ecm_sim* s, const DevState& d, int mode) {
             drop_graph(g);
             return 1;
         }
-        g.graph[h] = graph;
-        g.exec[h] = exec;
+        g.graph[0] = graph;
+        g.exec[0] = exec;
     }
     g.key = key;
     g.valid = true;
@@ -1252,24 +1338,28 @@ static int step_once(ecm_sim* s) {
         if (rc == 0) graph = true;
         else s->use_graph = false;           // capture refused: eager from now on (see graph_error)
     }
-    // half 0, the replan read-back event, half 1
+    // the whole step; its first half ends by publishing the events
     if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur].exec[0], s->stream));
-    else if ((rc = enqueue_device_half(s, d, mode, 0)) != 0) return rc;
-    HIPCHK(hipEventRecord(s->ev_replan, s->stream));
-    if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur].exec[1], s->stream));
-    else if ((rc = enqueue_device_half(s, d, mode, 1)) != 0) return rc;
-    const int peek = std::min(ecm_sim::kReplanPeek, s->prm.max_agents);
-    HIPCHK(hipEventSynchronize(s->ev_replan));
-    const uint32_t nr = s->h_counters[1];
-    std::vector<int> rid(s->h_replan_ids, s->h_replan_ids + std::min<uint32_t>(nr, peek));
-    std::vector<float2> rpos(s->h_replan_pos, s->h_replan_pos + std::min<uint32_t>(nr, peek));
-    if (nr > (uint32_t)peek) {
-        rid.resize(nr);
-        rpos.resize(nr);
+    else if ((rc = enqueue_device_half(s, d, mode, 0)) != 0 || (rc = enqueue_device_half(s, d, mode, 1)) != 0)
+        return rc;
+    const uint32_t peek = (uint32_t)std::min(ecm_sim::kEventPeek, s->prm.max_agents);
+    if ((rc = wait_published(s, ++s->step_seq)) != 0) return rc;
+    const uint32_t ne = s->h_xfer[1];
+    std::vector<int4> ev(std::min(ne, peek));          // events start at word 9 (not 16-byte aligned)
+    if (!ev.empty()) std::memcpy(ev.data(), s->h_xfer + 1 + 8, ev.size() * sizeof(int4));
+    if (ne > peek) {
+        ev.resize(ne);
         HIPCHK(hipStreamSynchronize(s->stream));
-        HIPCHK(hipMemcpy(rid.data() + peek, s->replan_list.p + peek, (nr - peek) * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(rpos.data() + peek, s->replan_pos.p + peek, (nr - peek) * sizeof(float2), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ev.data() + peek, s->xfer.p + 8 + 4 * (size_t)peek, (ne - peek) * sizeof(int4),
+                         hipMemcpyDeviceToHost));
     }
+    std::vector<int> rid, dead;
+    std::vector<float2> rpos;
+    for (const int4& e : ev) {
+        if (e.w) dead.push_back(e.x);
+        else { rid.push_back(e.x); rpos.push_back(make_float2(bits_to_float(e.y), bits_to_float(e.z))); }
+    }
+    const uint32_t nr = (uint32_t)rid.size(), nd = (uint32_t)dead.size();
     // order as the reference visits slots (Simulator.cpp:545)
     {
         std::vector<int> ord(nr);
@@ -1288,29 +1378,19 @@ static int step_once(ecm_sim* s) {
         rc = run_replans(s, rid, rpos);
         if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(s->stream));
+    // The step returns with its ORCA kernels possibly still running: every
+    // entry point that touches device data waits for the stream first, and
+    // the next step is ordered behind this one on the same stream.
     s->cur ^= 1;
     const int n_before = s->num_agents;
-    uint32_t nd = s->h_counters[0];
-    if (nd) {
-        std::vector<int> ids(nd);
-        HIPCHK(hipMemcpy(ids.data(), s->destroyed_list.p, nd * sizeof(int), hipMemcpyDeviceToHost));
-        std::sort(ids.begin(), ids.end());      // reference visits slots in ascending order
-        for (int id : ids) mark_destroyed(s, id);
-    }
+    std::sort(dead.begin(), dead.end());      // reference visits slots in ascending order
+    for (int id : dead) mark_destroyed(s, id);
     s->stat[0] += 1;
     s->stat[1] += (uint64_t)n_before;
-    s->stat[2] += s->h_counters[2];
-    s->stat[3] += s->h_counters[4];
     s->stat[4] += nr;
     s->stat[5] += nd;
-    s->stat[6] += s->h_counters[3];
-    for (int i = 0; i < 64; ++i) {
-        s->stat[7] += s->h_counters[8 + 4 * i];
-        s->stat[8] += s->h_counters[8 + 4 * i + 1];
-        s->stat[9] += s->h_counters[8 + 4 * i + 2];
-    }
     if (s->timing) {
+        HIPCHK(hipStreamSynchronize(s->stream));
         for (int i = 0; i < 4; ++i) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
@@ -1334,6 +1414,8 @@ extern "C" int ecm_sim_step(ecm_sim* s, int n_steps) {
 
 extern "C" int ecm_sim_set_graph(ecm_sim* s, int on) {
     if (!s) return fail(ECM_E_ARG, "null handle");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     s->use_graph = on != 0;
     if (on) s->graph_error.clear();
     return 0;
@@ -1373,6 +1455,8 @@ extern "C" int ecm_sim_get_tuning(ecm_sim* s, ecm_sim_tuning* out) {
 
 extern "C" int ecm_sim_set_tuning(ecm_sim* s, const ecm_sim_tuning* t) {
     if (!s || !t) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     int rc = check_tuning(*t);
     if (rc) return rc;
     apply_tuning(s, *t);
@@ -1388,6 +1472,7 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
     for (size_t e = 0; e < s->envs.size(); ++e)
         if (!s->envs[e]->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before autotune (environment %zu)", e);
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     auto trial = [&](const ecm_sim_tuning& t, float* ms) -> int {
         apply_tuning(s, t);
         int rc = step_once(s);
@@ -1456,6 +1541,7 @@ extern "C" int ecm_sim_read(ecm_sim* s, int c, void* dst, size_t bytes) {
     if (rc) return rc;
     if (bytes != el * (size_t)s->prm.max_agents) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * (size_t)s->prm.max_agents);
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
@@ -1470,6 +1556,7 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
     const size_t N = (size_t)s->prm.max_agents;
     if (bytes != el * N) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * N);
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     const float* f = (const float*)src;
@@ -1500,6 +1587,8 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
 
 extern "C" int ecm_sim_device_ptr(ecm_sim* s, int c, void** out) {
     if (!s || !out) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     size_t el;
     return component_buf(s, c, out, &el);
 }
@@ -1519,6 +1608,7 @@ extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
     if (!s || !nbr || !cnt) return fail(ECM_E_ARG, "null argument");
     if (s->n_keys == 0) return fail(ECM_E_STATE, "no step has run");
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     const size_t N = (size_t)s->prm.max_agents;
     DBuf<int> dn, dc;
     int rc = dn.alloc(N * s->prm.k);
@@ -1540,6 +1630,7 @@ extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
 extern "C" int ecm_sim_read_paths(ecm_sim* s, int* off, int* cnt, float* pool, size_t pool_cap, size_t* pool_used) {
     if (!s || !pool_used) return fail(ECM_E_ARG, "null argument");
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     *pool_used = s->path_used;
     const size_t N = (size_t)s->prm.max_agents;
     if (off) HIPCHK(hipMemcpy(off, s->path_off.p, N * sizeof(int), hipMemcpyDeviceToHost));
@@ -1560,12 +1651,33 @@ extern "C" int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n) {
 
 extern "C" int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n) {
     if (!s || !out || n < 0) return fail(ECM_E_ARG, "null argument");
-    for (int i = 0; i < n && i < ECM_SIM_N_STATS; ++i) out[i] = s->stat[i];
+    (void)hipSetDevice(s->dev);
+    if (int rc = settle(s)) return rc;
+    // device totals of the finished steps + the counters of the last step,
+    // which the next step's grid build has not folded in yet
+    unsigned long long acc[12];
+    uint32_t cur[8], ws[256];
+    HIPCHK(hipMemcpy(acc, s->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cur, s->xfer.p, sizeof(cur), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ws, s->wstats.p, sizeof(ws), hipMemcpyDeviceToHost));
+    uint64_t col[4] = {acc[8], acc[9], acc[10], acc[11]};
+    for (int i = 0; i < 256; ++i) col[i & 3] += ws[i];
+    uint64_t st[ECM_SIM_N_STATS];
+    for (int i = 0; i < ECM_SIM_N_STATS; ++i) st[i] = s->stat[i];
+    st[2] = acc[2] + cur[2];        // 3D-LP queue
+    st[3] = acc[4] + cur[4];        // locate fallbacks
+    st[6] = acc[3] + cur[3];        // obstacle overflow queue
+    st[7] = col[0];
+    st[8] = col[1];
+    st[9] = col[2];
+    for (int i = 0; i < n && i < ECM_SIM_N_STATS; ++i) out[i] = st[i];
     return 0;
 }
 
 extern "C" int ecm_sim_set_timing(ecm_sim* s, int on) {
     if (!s) return fail(ECM_E_ARG, "null handle");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     s->timing = on != 0;
     return 0;
 }
@@ -1590,6 +1702,7 @@ extern "C" int ecm_sim_set_learned(ecm_sim* s, ecm_mlp* m, float theta) {
     if (!s) return fail(ECM_E_ARG, "null handle");
     if (!std::isfinite(theta)) return fail(ECM_E_ARG, "theta must be finite");
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     if (m) {
         if (m->dev != s->dev) return fail(ECM_E_ARG, "network and simulator on different devices");
         int rc = ensure_learned_buffers(s);
@@ -1608,6 +1721,7 @@ extern "C" int ecm_sim_log_step(ecm_sim* s, ecm_mlp* m, int* n_logged) {
     if (m->dev != s->dev) return fail(ECM_E_ARG, "network and simulator on different devices");
     if (m->ds_cap < 1) return fail(ECM_E_STATE, "reserve dataset capacity first (ecm_mlp_dataset_reserve)");
     (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     int64_t before = 0;
     int rc = ecm_mlp_dataset_size(m, &before);
     if (rc) return rc;
@@ -1615,6 +1729,7 @@ extern "C" int ecm_sim_log_step(ecm_sim* s, ecm_mlp* m, int* n_logged) {
     rc = step_once(s);
     s->log_mlp = nullptr;
     if (rc) return rc;
+    if ((rc = settle(s)) != 0) return rc;
     int64_t after = 0;
     rc = ecm_mlp_dataset_size(m, &after);
     if (rc) return rc;
@@ -1635,6 +1750,8 @@ static bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n;
 // environment its world box, obstacles, ECM and agent count.
 extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
     if (!s || !path) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     const size_t N = (size_t)s->prm.max_agents;
     std::vector<float2> pos(N), vel(N), attr(N), pv(N);
     std::vector<float> rad(N), spd(N);
@@ -1678,6 +1795,8 @@ extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
 
 extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
     if (!s || !path) return fail(ECM_E_ARG, "null argument");
+    (void)hipSetDevice(s->dev);
+    if (int rc_ = settle(s)) return rc_;
     FILE* f = fopen(path, "rb");
     if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
     uint32_t hdr[4];

@@ -89,13 +89,19 @@ struct DevState {
     const int* nbr_id_prev;   // the previous step's nbr_id
 
     // ---- events ----
-    int* destroyed_list;  uint32_t* destroyed_n;   // DestroyAgent requests
-    int* replan_list;     uint32_t* replan_n;      // UpdatePath requests
-    float2* replan_pos;                            // step-start position of each request
+    // One list per step, read back by the host after the neighbour pass:
+    // (id, x bits, y bits, 0) asks UpdatePath from the step-start position
+    // (x, y); (id, 0, 0, 1) is a DestroyAgent.
+    int4* events;         uint32_t* events_n;
     uint32_t* stats;      // per step: [0] 3D-LP queue length, [1] obstacle-overflow queue
                           // length, [2] locate fallbacks to the exact-y list, [3..5] unused
-    uint32_t* wstats;     // per step, [64][4] slots summed by the host: learned proposals
-                          // tried, accepted, constraints proposed, unused
+    uint32_t* wstats;     // per step, [64][4] slots: learned proposals tried, accepted,
+                          // constraints proposed, unused
+    unsigned long long* acc;   // [16] totals of the finished steps; [12] steps started
+    // host-mapped [1 + 8 + 4 * event_peek] words: [0] number of the last
+    // step that published, then its counters and first events
+    uint32_t* host_xfer;
+    uint32_t event_peek;
     int* ovf_agent;         // overflow queue: sorted positions
     // 3D-LP queue: agents whose 2D LP failed, with their constraints
     int* q3d_agent;         // sorted position

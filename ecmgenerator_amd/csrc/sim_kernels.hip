@@ -50,7 +50,28 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 // ------------------------------------------------------------ grid build
-__global__ void k_grid_count(DevState s) {
+// Start of a step (block 0 of k_grid_count, 256 threads): add the
+// previous step's counters (8 words) and the column sums of the
+// per-wavefront slots ([64][4] words) to the 64-bit totals (acc[0..7] and
+// acc[8..11]), count the step (acc[12]), then clear both for this step.
+// No other block of the grid touches them.
+__device__ __forceinline__ void step_begin(uint32_t* counters, uint32_t* wstats, unsigned long long* acc) {
+    __shared__ uint32_t col[4];
+    const int t = threadIdx.x;
+    if (t < 4) col[t] = 0u;
+    __syncthreads();
+    const uint32_t w = wstats[t];
+    if (w) atomicAdd(&col[t & 3], w);
+    const uint32_t c = t < 8 ? counters[t] : 0u;
+    __syncthreads();
+    if (t < 8) { acc[t] += c; counters[t] = 0u; }
+    if (t < 4) acc[8 + t] += col[t];
+    if (t == 0) acc[12] += 1ull;
+    wstats[t] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_grid_count(DevState s) {
+    if (blockIdx.x == 0) step_begin(s.events_n, s.wstats, s.acc);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
     if (!s.active[i]) { s.key[i] = kInvalidKey; s.slot_of[i] = -1; return; }
@@ -444,10 +465,9 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
 // lane in dynamic LDS (kKnnBuf * blockDim.x ints per block).
 static constexpr int kKnnBuf = 16;
 
-// K == k: the k-th neighbour is the last register of the list.
-template <int K>
-__global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
-    extern __shared__ int knn_lds[];
+// One sorted agent: attraction point, steering, k nearest.
+template <int K>   // K == k: the k-th neighbour is the last register of the list
+__device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) {
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
@@ -474,17 +494,16 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
         if (irm_attraction(E, pos, path, np, ap, fb)) attr = ap;
         else {
             // UpdatePath(e, currentPosition, goal) with the step-start position
-            uint32_t q = atomicAdd(s.replan_n, 1u);
-            s.replan_list[q] = id;
-            s.replan_pos[q] = make_float2(pos.x, pos.y);
+            uint32_t q = atomicAdd(s.events_n, 1u);
+            s.events[q] = make_int4(id, __float_as_int(pos.x), __float_as_int(pos.y), 0);
         }
     }
     s.attraction[id] = make_float2(attr.x, attr.y);
     if (fb) atomicAdd(&s.stats[2], 1u);
     if (!alive) {
         s.active[id] = 0;                                   // DestroyAgent
-        uint32_t q = atomicAdd(s.destroyed_n, 1u);
-        s.destroyed_list[q] = id;
+        uint32_t q = atomicAdd(s.events_n, 1u);
+        s.events[q] = make_int4(id, 0, 0, 1);
         s.nbr_cnt[t] = 0;
         for (int i = 0; i < K; ++i) s.nbr_id[(size_t)i * s.n_slots + t] = -1;
         return;
@@ -509,6 +528,31 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
         s.nbr_id[(size_t)i * s.n_slots + t] = i < cnt ? nid[i] : -1;
     }
     s.nbr_cnt[t] = cnt;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
+    extern __shared__ int knn_lds[];
+    attr_knn_agent<K>(s, knn_lds);
+}
+
+// Copies the step's counters and its first event_peek events (words
+// [0, 8 + 4 * min(events, peek)) of the block that events_n heads) to
+// host_xfer[1..], then, after a system-scope fence, the step number
+// (acc[12]) to host_xfer[0]: the host polls that word and reads the rest
+// once it matches.  One block, launched after k_attr_knn: the kernel
+// boundary makes every event visible (a fence in each block of k_attr_knn
+// would write back the L2 of its XCD once per block).
+__global__ __launch_bounds__(256) void k_publish(DevState s) {
+    const uint32_t* xfer = s.events_n;
+    const uint32_t n = 8u + 4u * min(xfer[0], s.event_peek);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s.host_xfer[1 + i] = xfer[i];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *(volatile uint32_t*)s.host_xfer = (uint32_t)s.acc[12];
+    }
 }
 
 // ------------------------------------------------------------------- ORCA
@@ -1242,6 +1286,10 @@ hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
 #undef ECM_KNN_CASE
         default: return hipErrorInvalidValue;
     }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the step's events go to the host as soon as the neighbour pass is done
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, st, s);
     return hipGetLastError();
 }
 
