@@ -1147,7 +1147,10 @@ __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     __shared__ float4 lds[(MAXO + K) * 64];
     const uint32_t nq = s.stats[1];
     const int tid = threadIdx.x;
-    for (uint32_t i = blockIdx.x * 64 + tid; i < nq; i += gridDim.x * 64)
+    // entries go lane-major across the blocks (entry i: lane i / gridDim,
+    // block i % gridDim), so a short queue spreads over many CUs instead of
+    // filling the lanes of a few divergent wavefronts
+    for (uint32_t i = tid * gridDim.x + blockIdx.x; i < nq; i += gridDim.x * 64)
         orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, tid);
 }
 
@@ -1159,7 +1162,8 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     __shared__ float4 lds[(MAXO + 2 * K) * 64];
     const uint32_t nq = s.stats[0];
     const int tid = threadIdx.x;
-    for (uint32_t q = blockIdx.x * 64 + tid; q < nq; q += gridDim.x * 64) {
+    // entries go lane-major across the blocks, as in k_orca_ovf
+    for (uint32_t q = tid * gridDim.x + blockIdx.x; q < nq; q += gridDim.x * 64) {
         const int t = s.q3d_agent[q];
         int4 info = s.q3d_info[q];
         const float4* src = s.q3d_con + (size_t)q * s.con_stride;
