@@ -134,8 +134,9 @@ struct ecm_sim {
     DBuf<uint32_t> key, rank, cell_count, cell_start, bsum;
     DBuf<float4> sorted_pr;
     DBuf<float2> sorted_v;
-    DBuf<int> nbr, nbr_cnt;
-    DBuf<int> slot_of[2], nbr_id[2];   // neighbour-search seeds, by step parity
+    DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
+    DBuf<int> slot_of[2];              // agent -> sorted position, by step parity
+    DBuf<int> fwd, prev_of;            // previous <-> this step's sorted positions
 
     // Per-step counters and the event list share one device block, so one
     // copy brings both back: words [0..7] are the counters ([0] events,
@@ -256,8 +257,9 @@ static void free_all(ecm_sim* s) {
     s->radius.release(); s->pref_speed.release(); s->active.release();
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
-    s->sorted_pr.release(); s->sorted_v.release(); s->nbr.release(); s->nbr_cnt.release();
-    for (int b = 0; b < 2; ++b) { s->slot_of[b].release(); s->nbr_id[b].release(); }
+    s->sorted_pr.release(); s->sorted_v.release();
+    for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); s->slot_of[b].release(); }
+    s->fwd.release(); s->prev_of.release();
     s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     for (auto& e : s->envs) e->release();
@@ -353,9 +355,13 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->rank.alloc(N);
     if (!rc) rc |= s->sorted_pr.alloc(N);
     if (!rc) rc |= s->sorted_v.alloc(N);
-    if (!rc) rc |= s->nbr.alloc(N * (size_t)p->k);
-    if (!rc) rc |= s->nbr_cnt.alloc(N);
-    for (int b = 0; b < 2 && !rc; ++b) { rc |= s->slot_of[b].alloc(N); rc |= s->nbr_id[b].alloc(N * (size_t)p->k); }
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc |= s->nbr[b].alloc(N * (size_t)p->k);
+        rc |= s->nbr_cnt[b].alloc(N);
+        rc |= s->slot_of[b].alloc(N);
+    }
+    if (!rc) rc |= s->fwd.alloc(N);
+    if (!rc) rc |= s->prev_of.alloc(N);
     if (!rc) rc |= s->xfer.alloc(8 + 4 * N);
     if (!rc) rc |= s->wstats.alloc(256);
     if (!rc) rc |= s->acc.alloc(16);
@@ -379,10 +385,12 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc |= zero_dev(s->pref_vel.p, N * sizeof(float2), s->stream);
         rc |= zero_dev(s->active.p, N, s->stream);
         rc |= zero_dev(s->path_cnt.p, N * sizeof(int), s->stream);
-        for (int b = 0; b < 2; ++b) {      // all -1: no seeds before the first step
+        for (int b = 0; b < 2; ++b) {      // no seeds before the first step
             rc |= fill_dev_bytes(s->slot_of[b].p, 0xff, N * sizeof(int), s->stream);
-            rc |= fill_dev_bytes(s->nbr_id[b].p, 0xff, N * (size_t)p->k * sizeof(int), s->stream);
+            rc |= zero_dev(s->nbr_cnt[b].p, N * sizeof(int), s->stream);
         }
+        rc |= fill_dev_bytes(s->fwd.p, 0xff, N * sizeof(int), s->stream);
+        rc |= fill_dev_bytes(s->prev_of.p, 0xff, N * sizeof(int), s->stream);
         rc |= zero_dev(s->path_off.p, N * sizeof(int), s->stream);
         rc |= zero_dev(s->radius.p, N * sizeof(float), s->stream);
         rc |= zero_dev(s->pref_speed.p, N * sizeof(float), s->stream);
@@ -1052,9 +1060,11 @@ static DevState make_state(ecm_sim* s) {
     d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
     d.n_keys = s->n_keys;
     d.key = s->key.p; d.rank = s->rank.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
-    d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p; d.nbr = s->nbr.p; d.nbr_cnt = s->nbr_cnt.p;
+    d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p;
+    d.nbr = s->nbr[s->cur].p; d.nbr_cnt = s->nbr_cnt[s->cur].p;
+    d.nbr_prev = s->nbr[s->cur ^ 1].p; d.nbr_cnt_prev = s->nbr_cnt[s->cur ^ 1].p;
     d.slot_of = s->slot_of[s->cur].p; d.slot_of_prev = s->slot_of[s->cur ^ 1].p;
-    d.nbr_id = s->nbr_id[s->cur].p; d.nbr_id_prev = s->nbr_id[s->cur ^ 1].p;
+    d.fwd = s->fwd.p; d.prev_of = s->prev_of.p;
     d.events = reinterpret_cast<int4*>(s->xfer.p + 8); d.events_n = s->xfer.p;
     d.stats = s->xfer.p + 2;
     d.wstats = s->wstats.p;
@@ -1616,8 +1626,10 @@ extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(dn.p, 0xff, N * s->prm.k * sizeof(int), s->stream));
     HIPCHK(hipMemsetAsync(dc.p, 0, N * sizeof(int), s->stream));
-    // the last step read buffer cur^1; sorted arrays are still those of that step
+    // the last step ran with parity cur^1; sorted arrays are still those of that step
     DevState d = make_state(s);
+    d.nbr = s->nbr[s->cur ^ 1].p;
+    d.nbr_cnt = s->nbr_cnt[s->cur ^ 1].p;
     HIPCHK(launch_export_nbrs(d, dn.p, dc.p, s->stream));
     HIPCHK(hipMemcpyAsync(nbr, dn.p, N * s->prm.k * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(cnt, dc.p, N * sizeof(int), hipMemcpyDeviceToHost, s->stream));

@@ -81,12 +81,14 @@ struct DevState {
     float2* sorted_v;       // [n_slots] (vx, vy)
     int* nbr;               // [k][n_slots] neighbour sorted positions (SoA)
     int* nbr_cnt;           // [n_slots]   by sorted position
-    // neighbour-search seeds: the previous step's neighbours of each agent
-    // bound its k-th distance before the search starts (double-buffered)
+    // neighbour-search seeds: each agent's neighbours of the previous step
+    // bound its k-th distance before the search starts
     int* slot_of;             // [n_slots] sorted position of each agent, -1 off the grid
-    const int* slot_of_prev;  // the same map of the previous step, null: no seeds
-    int* nbr_id;              // [k][n_slots] neighbour agent ids by sorted position, -1 pad
-    const int* nbr_id_prev;   // the previous step's nbr_id
+    const int* slot_of_prev;  // the same map of the previous step
+    const int* nbr_prev;      // the previous step's nbr and nbr_cnt (by its sorted positions)
+    const int* nbr_cnt_prev;
+    int* fwd;                 // [n_slots] previous sorted position -> this step's, -1: left the grid
+    int* prev_of;             // [n_slots] this step's sorted position -> previous, -1: none
 
     // ---- events ----
     // One list per step, read back by the host after the neighbour pass:

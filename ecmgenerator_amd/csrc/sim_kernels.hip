@@ -74,7 +74,13 @@ __global__ __launch_bounds__(256) void k_grid_count(DevState s) {
     if (blockIdx.x == 0) step_begin(s.events_n, s.wstats, s.acc);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
-    if (!s.active[i]) { s.key[i] = kInvalidKey; s.slot_of[i] = -1; return; }
+    if (!s.active[i]) {
+        s.key[i] = kInvalidKey;
+        s.slot_of[i] = -1;
+        const int q = s.slot_of_prev[i];
+        if (q >= 0) s.fwd[q] = -1;           // on the grid last step, gone now
+        return;
+    }
     float2 p = s.pos_in[i];
     const EnvDesc& E = env_of(s, i);
     int cx = grid_cell(p.x, E.ag.x0, E.ag.inv_cell, E.ag.side);
@@ -163,6 +169,19 @@ __global__ void k_grid_scatter(DevState s) {
     float2 p = s.pos_in[i];
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
     s.sorted_v[slot] = s.vel_in[i];
+}
+
+// Maps between the previous and this step's sorted positions, in sorted
+// order: the agent order barely changes from step to step, so the writes
+// to fwd land close together (agents that left the grid were cleared by
+// k_grid_count).
+__global__ void k_seed_map(DevState s) {
+    const uint32_t n_active = s.cell_start[s.n_keys];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)n_active) return;
+    const int q = s.slot_of_prev[__float_as_int(s.sorted_pr[t].w)];
+    s.prev_of[t] = q;
+    if (q >= 0) s.fwd[q] = t;
 }
 
 // ------------------------------------------------------------- ECM query
@@ -304,57 +323,71 @@ __device__ __forceinline__ bool nb_better(float d, int id, float d2, int id2) {
     return (d < d2) | ((d == d2) & (id < id2));      // bitwise: no branches
 }
 
-// Insert (dd, ii) into the ascending register list (d, id) of length K:
-// one comparison per entry (the flags are monotone along the list), then a
-// predicated shift; no dynamic indexing.
+// Agent id at a sorted position (INT_MAX for an empty list entry, -1).
+__device__ __forceinline__ int id_at(const DevState& s, int slot) {
+    const int v = __float_as_int(s.sorted_pr[slot < 0 ? 0 : slot].w);
+    return slot < 0 ? INT_MAX : v;
+}
+
+// Insert candidate (dd, id ii, sorted position ss) into the ascending list
+// (d, sl) of length K, ordered by (d2, id): one comparison per entry (the
+// flags are monotone along the list), then a predicated shift; no dynamic
+// indexing.  The list keeps sorted positions, not ids: ids are loaded only
+// when dd ties an entry's distance.
 template <int K>
-__device__ __forceinline__ void nb_insert(float (&d)[K], int (&id)[K], float dd, int ii) {
-    bool b[K];
+__device__ __forceinline__ void nb_insert(const DevState& s, float (&d)[K], int (&sl)[K], float dd, int ii, int ss) {
+    bool tie = false;
 #pragma unroll
-    for (int i = 0; i < K; ++i) b[i] = nb_better(dd, ii, d[i], id[i]);
+    for (int i = 0; i < K; ++i) tie |= dd == d[i];
+    bool b[K];
+    if (!tie) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) b[i] = dd < d[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i) b[i] = nb_better(dd, ii, d[i], dd == d[i] ? id_at(s, sl[i]) : 0);
+    }
 #pragma unroll
     for (int i = K - 1; i > 0; --i) {
         d[i] = b[i - 1] ? d[i - 1] : (b[i] ? dd : d[i]);
-        id[i] = b[i - 1] ? id[i - 1] : (b[i] ? ii : id[i]);
+        sl[i] = b[i - 1] ? sl[i - 1] : (b[i] ? ss : sl[i]);
     }
     d[0] = b[0] ? dd : d[0];
-    id[0] = b[0] ? ii : id[0];
+    sl[0] = b[0] ? ss : sl[0];
 }
 
-// Upper bound on the squared k-th neighbour distance of agent `id` at p:
-// the largest squared distance to the k agents it had as neighbours in the
-// previous step, when all of them are still on the grid of its environment
-// and none is the agent itself (any k distinct candidates bound the k-th
-// distance from above).  INFINITY when there is no such seed.
+// Upper bound on the squared k-th neighbour distance of the agent at sorted
+// position t (id, at p): the largest squared distance to the k agents it
+// had as neighbours in the previous step, taken to this step's sorted
+// positions through fwd, when all of them are still on the grid of its
+// environment and none is the agent itself (any k distinct candidates bound
+// the k-th distance from above; fwd is one-to-one).  INFINITY when there is
+// no such seed.
 template <int K>   // K == k
-__device__ __forceinline__ float knn_seed_bound(const DevState& s, int id, V2 p) {
-    if (!s.slot_of_prev) return INFINITY;
-    const int q = s.slot_of_prev[id];
-    if (q < 0 || q >= s.n_slots) return INFINITY;
+__device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id, V2 p) {
+    const int q = s.prev_of[t];
+    if (q < 0 || q >= s.n_slots || s.nbr_cnt_prev[q] < K) return INFINITY;
     const int env = s.agent_env ? (int)s.agent_env[id] : 0;
-    int nid[K];
+    int ns[K];
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        nid[j] = s.nbr_id_prev[(size_t)j * s.n_slots + q];
-        ok &= nid[j] >= 0 && nid[j] < s.n_slots;
+        const int os = s.nbr_prev[(size_t)j * s.n_slots + q];
+        const bool in = os >= 0 && os < s.n_slots;
+        const int f = s.fwd[in ? os : 0];          // out-of-range entries read slot 0
+        ns[j] = in ? f : -1;
     }
-    if (!ok) return INFINITY;
-    int sl[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        sl[j] = s.slot_of[nid[j]];
-        ok &= sl[j] >= 0;
-        if (s.agent_env) ok &= (int)s.agent_env[nid[j]] == env;
-    }
+    for (int j = 0; j < K; ++j) ok &= ns[j] >= 0 && ns[j] < s.n_slots;
     if (!ok) return INFINITY;
     float bound = 0.0f;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const float4 c = s.sorted_pr[sl[j]];
+        const float4 c = s.sorted_pr[ns[j]];
         const float dx = c.x - p.x, dy = c.y - p.y;
         const float d2 = dx * dx + dy * dy;
         ok &= d2 > kEps;
+        if (s.agent_env) ok &= (int)s.agent_env[__float_as_int(c.w)] == env;
         bound = fmaxf(bound, d2);
     }
     return ok ? bound : INFINITY;
@@ -370,17 +403,16 @@ __device__ __forceinline__ float knn_seed_bound(const DevState& s, int id, V2 p)
 // list at the end of each ring or when the buffer is full, so the lanes of
 // a wave rarely wait on each other's list inserts.  The list is ordered by
 // (d2, id), so the result does not depend on visiting order; `bound` (from
-// knn_seed_bound) only prunes.  The list holds agent ids; their sorted
-// positions come from slot_of afterwards.
+// knn_seed_bound) only prunes.  The list holds sorted positions.
 template <int K, int B>   // K == k
 __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2 p, float bound,
-                                         int* buf, int bstride, int (&out_id)[K]) {
+                                         int* buf, int bstride, int (&sl)[K]) {
     const GridDesc& g = E.ag;
     const int cx = grid_cell(p.x, g.x0, g.inv_cell, g.side);
     const int cy = grid_cell(p.y, g.y0, g.inv_cell, g.side);
     float d[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) { d[i] = INFINITY; out_id[i] = INT_MAX; }
+    for (int i = 0; i < K; ++i) { d[i] = INFINITY; sl[i] = -1; }
     int n = 0, nb = 0;
     const float guard = 2e-3f;
     // thr: candidates with d2 <= thr are kept; it is the bound until the
@@ -400,8 +432,10 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
             const float dx = c.x - p.x, dy = c.y - p.y;
             const float d2 = dx * dx + dy * dy;
             const int cid = __float_as_int(c.w);
-            if (nb_better(d2, cid, d[K - 1], out_id[K - 1])) {
-                nb_insert<K>(d, out_id, d2, cid);
+            bool better = d2 < d[K - 1];
+            if (!better && d2 == d[K - 1]) better = cid < id_at(s, sl[K - 1]);
+            if (better) {
+                nb_insert<K>(s, d, sl, d2, cid, j);
                 n = min(n + 1, K);
             }
         }
@@ -505,7 +539,6 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
         uint32_t q = atomicAdd(s.events_n, 1u);
         s.events[q] = make_int4(id, 0, 0, 1);
         s.nbr_cnt[t] = 0;
-        for (int i = 0; i < K; ++i) s.nbr_id[(size_t)i * s.n_slots + t] = -1;
         return;
     }
     // ApplySteeringForce (Simulator.cpp:638-657)
@@ -514,19 +547,12 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
     s.pref_vel[id] = make_float2(des.x, des.y);
 
     // k nearest (replaces KDTree::KNearestAgents)
-    int slot[K], nid[K];
-    const float bound = knn_seed_bound<K>(s, id, pos);
-    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, nid);
+    int slot[K];
+    const float bound = knn_seed_bound<K>(s, t, id, pos);
+    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, slot);
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const int sl = s.slot_of[i < cnt ? nid[i] : 0];   // unused entries read slot 0
-        slot[i] = i < cnt ? sl : -1;
-    }
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
+    for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
-        s.nbr_id[(size_t)i * s.n_slots + t] = i < cnt ? nid[i] : -1;
-    }
     s.nbr_cnt[t] = cnt;
 }
 
@@ -1247,6 +1273,7 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb, s.cell_start, G);
     hipLaunchKernelGGL(k_scan_add, dim3(cdiv(G, 256)), dim3(256), 0, st, s.cell_start, (const uint32_t*)bsum, G);
     hipLaunchKernelGGL(k_grid_scatter, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_seed_map, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
     return hipGetLastError();
 }
 
