@@ -340,12 +340,12 @@ __device__ __forceinline__ void nb_insert(const DevState& s, float (&d)[K], int 
 #pragma unroll
     for (int i = 0; i < K; ++i) tie |= dd == d[i];
     bool b[K];
-    if (!tie) {
 #pragma unroll
-        for (int i = 0; i < K; ++i) b[i] = dd < d[i];
-    } else {
+    for (int i = 0; i < K; ++i) b[i] = dd < d[i];
+    if (tie) {
 #pragma unroll
-        for (int i = 0; i < K; ++i) b[i] = nb_better(dd, ii, d[i], dd == d[i] ? id_at(s, sl[i]) : 0);
+        for (int i = 0; i < K; ++i)
+            if (dd == d[i]) b[i] = ii < id_at(s, sl[i]);
     }
 #pragma unroll
     for (int i = K - 1; i > 0; --i) {
