@@ -146,6 +146,11 @@ int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_xy, const f
 int ecm_sim_env_set_replan_callback(ecm_sim* s, int env, ecm_replan_fn fn, void* user);
 /* Environment of every slot (max_agents entries, -1 for free slots). */
 int ecm_sim_agent_envs(ecm_sim* s, int* env);
+/* Runs n_steps steps (Simulator::Update, above).  Returns once the host part
+ * of the last step is done (events read, replans planned and
+ * uploaded, destroyed agents booked); its last kernels may still be running
+ * on the handle's stream.  Every other call that reads or replaces device
+ * data waits for the stream first. */
 int ecm_sim_step(ecm_sim* s, int n_steps);
 /* Wait for the handle's stream. */
 int ecm_sim_sync(ecm_sim* s);
