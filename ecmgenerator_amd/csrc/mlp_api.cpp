@@ -69,10 +69,13 @@ extern "C" int ecm_mlp_create(uint64_t seed, ecm_mlp** out) {
     return 0;
 }
 
+// A simulator step returns before its kernels finish, and a learned step
+// reads the packed weights (frags, pbias) on the simulator's stream.  The
+// calls that free or rewrite them therefore wait for the whole device.
 extern "C" int ecm_mlp_destroy(ecm_mlp* m) {
     if (!m) return 0;
     (void)hipSetDevice(m->dev);
-    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    (void)hipDeviceSynchronize();
     for (auto* b : {&m->params, &m->grads, &m->adam_m, &m->adam_v, &m->pbias, &m->z1, &m->z2, &m->z3, &m->y, &m->dyf,
                     &m->dh, &m->dzf, &m->loss})
         b->release();
@@ -98,6 +101,7 @@ extern "C" int ecm_mlp_set_params(ecm_mlp* m, const float* in, int n) {
     for (int i = 0; i < n; ++i)
         if (!std::isfinite(in[i])) return fail(ECM_E_ARG, "parameter %d is not finite", i);
     (void)hipSetDevice(m->dev);
+    HIPCHK(hipDeviceSynchronize());          // a simulator step may still read the packed weights
     HIPCHK(hipMemcpyAsync(m->params.p, in, kNParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
     HIPCHK(launch_to_bf16(m->params.p, m->wbf.p, kNParams, m->stream));
     HIPCHK(launch_fill_f32(m->adam_m.p, 0.0f, kNParams, m->stream));
@@ -282,6 +286,7 @@ extern "C" int ecm_mlp_train(ecm_mlp* m, int epochs, int batch, float lr, float 
                              uint64_t seed, float* losses) {
     if (!m || epochs < 0 || batch < 64 || !(lr > 0.0f)) return fail(ECM_E_ARG, "bad training arguments");
     (void)hipSetDevice(m->dev);
+    HIPCHK(hipDeviceSynchronize());          // a simulator step may still read the packed weights
     int64_t n = 0;
     int rc = ecm_mlp_dataset_size(m, &n);
     if (rc) return rc;
