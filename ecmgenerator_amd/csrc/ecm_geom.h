@@ -195,4 +195,16 @@ ECM_HD V2 rotate(V2 v, float rad) {
     return mk(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
 }
 
+// rotate(v, rad) and rotate(v, -rad) from one sincos_f: sincos_f is odd /
+// even in x bit for bit (rintf, the two-constant reduction and both
+// polynomials are symmetric, and the quadrant swap maps q to -q), and
+// v.y * -sn = -(v.y * sn), a - (-b) = a + b, (-a) + b = b - a exactly, so
+// both results equal the two separate rotations bit for bit.
+ECM_HD void rotate_pm(V2 v, float rad, V2& plus, V2& minus) {
+    V2 sc = sincos_f(rad);
+    float cs = sc.y, sn = sc.x;
+    plus = mk(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    minus = mk(v.x * cs + v.y * sn, v.y * cs - v.x * sn);
+}
+
 }  // namespace ecm

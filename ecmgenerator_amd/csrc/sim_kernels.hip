@@ -731,8 +731,8 @@ __device__ void agent_constraint(const DevState& s, V2 position, V2 velocity, fl
         return;
     }
     float half = atan_f(VORadius / VOPosLength);
-    V2 VOLeftLeg = rotate(VOPos, half);
-    V2 VORightLeg = rotate(VOPos, -half);
+    V2 VOLeftLeg, VORightLeg;
+    rotate_pm(VOPos, half, VOLeftLeg, VORightLeg);     // rotate(VOPos, +-half)
     float sqFromCentre = sq_dist(VOPos, relVel);
     bool below = det(VOLeftLeg - VOPos, relVel - VOPos) > 0;
     if (below) {
