@@ -27,7 +27,6 @@ hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
-hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
                                 const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
                                 float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,

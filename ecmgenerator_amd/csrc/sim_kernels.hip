@@ -1256,11 +1256,6 @@ __global__ void k_export_nbrs(DevState s, int* out, int* out_cnt) {
     out_cnt[id] = cnt;
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // ------------------------------------------------------------- launchers
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 static inline int round8(int n) { return (n + 7) & ~7; }
@@ -1343,12 +1338,6 @@ hipError_t launch_features(const DevState& s, hipStream_t st) {
 
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st) {
     hipLaunchKernelGGL(k_export_nbrs, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s, out, out_cnt);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int n, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
