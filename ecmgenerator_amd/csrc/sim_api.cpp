@@ -146,7 +146,8 @@ struct ecm_sim {
     DBuf<uint32_t> wstats;                   // [64][4]
     DBuf<unsigned long long> acc;            // [16] totals (folded in by k_grid_count); [12] steps run
     // host-mapped, 1 + 8 + 4 * kEventPeek words: [0] the number of the last
-    // step that published, then its counters and first events (k_publish)
+    // step that published, then its counters and first events (written by
+    // the last block of k_attr_knn)
     uint32_t* h_xfer = nullptr;
     uint32_t* h_xfer_dev = nullptr;          // the same memory as the kernels see it
     uint32_t step_seq = 0;                   // steps launched (matches acc[12])
@@ -1070,6 +1071,7 @@ static DevState make_state(ecm_sim* s) {
     d.acc = s->acc.p;
     d.host_xfer = s->h_xfer_dev;
     d.event_peek = (uint32_t)std::min(ecm_sim::kEventPeek, s->prm.max_agents);
+    d.blocks_done = s->xfer.p + 5;
     d.ovf_agent = s->ovf_agent.p;
     d.feat = s->feat.p;
     d.mlp_mask = s->mlp_mask.p;
@@ -1237,9 +1239,9 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
         HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
         if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
         HIPCHK(launch_attr_knn(d, s->tune.knn_block, s->stream));
-        // replan and destroy events are final once k_attr_knn is done;
-        // k_publish (launched right after it) writes them to host memory,
-        // and the host handles them while k_orca runs
+        // replan and destroy events are final once k_attr_knn is done; its
+        // last block writes them to host memory, and the host handles them
+        // while k_orca runs
         return 0;
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
@@ -1255,7 +1257,7 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
     return 0;
 }
 
-// Waits until step `seq` has published its events (k_publish writes the
+// Waits until step `seq` has published its events (k_attr_knn writes the
 // step number last, after a system-scope fence).  The stream is polled
 // now and then so that a step that ends without publishing is an error,
 // not a hang.

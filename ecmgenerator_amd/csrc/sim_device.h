@@ -104,6 +104,7 @@ struct DevState {
     // step that published, then its counters and first events
     uint32_t* host_xfer;
     uint32_t event_peek;
+    uint32_t* blocks_done;     // blocks of k_attr_knn finished this step
     int* ovf_agent;         // overflow queue: sorted positions
     // 3D-LP queue: agents whose 2D LP failed, with their constraints
     int* q3d_agent;         // sorted position

@@ -499,12 +499,13 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
 // lane in dynamic LDS (kKnnBuf * blockDim.x ints per block).
 static constexpr int kKnnBuf = 16;
 
-// One sorted agent: attraction point, steering, k nearest.
+// One sorted agent: attraction point, steering, k nearest.  Returns whether
+// it appended an event (a replan or a destroy), after fencing it device-wide.
 template <int K>   // K == k: the k-th neighbour is the last register of the list
-__device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) {
+__device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) {
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= (int)n_active) return;
+    if (t >= (int)n_active) return false;
     float4 pr = s.sorted_pr[t];
     const int id = __float_as_int(pr.w);
     const V2 pos = mk(pr.x, pr.y);
@@ -518,7 +519,7 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
     float dGoal = gx * gx + gy * gy;
     float2 a0 = s.attraction[id];
     V2 attr = mk(a0.x, a0.y);
-    bool alive = true;
+    bool alive = true, event = false;
     uint32_t fb = 0;
     if (dGoal < s.arrival_r2) {
         attr = mk(gg.x, gg.y);
@@ -530,6 +531,7 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
             // UpdatePath(e, currentPosition, goal) with the step-start position
             uint32_t q = atomicAdd(s.events_n, 1u);
             s.events[q] = make_int4(id, __float_as_int(pos.x), __float_as_int(pos.y), 0);
+            event = true;
         }
     }
     s.attraction[id] = make_float2(attr.x, attr.y);
@@ -539,7 +541,8 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
         uint32_t q = atomicAdd(s.events_n, 1u);
         s.events[q] = make_int4(id, 0, 0, 1);
         s.nbr_cnt[t] = 0;
-        return;
+        __threadfence();
+        return true;
     }
     // ApplySteeringForce (Simulator.cpp:638-657)
     const float speed = s.pref_speed[id];
@@ -554,31 +557,44 @@ __device__ __forceinline__ void attr_knn_agent(const DevState& s, int* knn_lds) 
     for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
     s.nbr_cnt[t] = cnt;
+    if (event) __threadfence();
+    return event;
+}
+
+// The block that finishes last copies the step's counters and its first
+// event_peek events (words [0, 8 + 4 * min(events, peek)) of the block that
+// events_n heads) to host_xfer[1..], then, after a system-scope fence, the
+// step number (acc[12]) to host_xfer[0]: the host polls that word and reads
+// the rest once it matches.  Only the threads that appended an event fence
+// (device scope, before the block barrier), so the rare events cost the
+// rare fences; an agent-scope fence in every block would write back its
+// XCD's L2 each time.  The last block reads with device-scope atomic loads.
+__device__ __forceinline__ void publish_if_last(const DevState& s) {
+    __shared__ uint32_t last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(s.blocks_done, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const uint32_t* xfer = s.events_n;
+    const uint32_t ne = __hip_atomic_load(xfer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t n = 8u + 4u * min(ne, s.event_peek);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        s.host_xfer[1 + i] = __hip_atomic_load(xfer + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *s.blocks_done = 0u;
+        __threadfence_system();
+        *(volatile uint32_t*)s.host_xfer = (uint32_t)s.acc[12];
+    }
 }
 
 template <int K>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     extern __shared__ int knn_lds[];
     attr_knn_agent<K>(s, knn_lds);
-}
-
-// Copies the step's counters and its first event_peek events (words
-// [0, 8 + 4 * min(events, peek)) of the block that events_n heads) to
-// host_xfer[1..], then, after a system-scope fence, the step number
-// (acc[12]) to host_xfer[0]: the host polls that word and reads the rest
-// once it matches.  One block, launched after k_attr_knn: the kernel
-// boundary makes every event visible (a fence in each block of k_attr_knn
-// would write back the L2 of its XCD once per block).
-__global__ __launch_bounds__(256) void k_publish(DevState s) {
-    const uint32_t* xfer = s.events_n;
-    const uint32_t n = 8u + 4u * min(xfer[0], s.event_peek);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s.host_xfer[1 + i] = xfer[i];
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        *(volatile uint32_t*)s.host_xfer = (uint32_t)s.acc[12];
-    }
+    publish_if_last(s);
 }
 
 // ------------------------------------------------------------------- ORCA
@@ -1312,10 +1328,6 @@ hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
 #undef ECM_KNN_CASE
         default: return hipErrorInvalidValue;
     }
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // the step's events go to the host as soon as the neighbour pass is done
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, st, s);
     return hipGetLastError();
 }
 
