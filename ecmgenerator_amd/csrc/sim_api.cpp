@@ -23,6 +23,8 @@
 
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
+hipError_t launch_unsort(const DevState& s, float2* pos, float2* vel, float2* attraction, float2* pref_vel,
+                         float2* v_orca, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
@@ -121,7 +123,9 @@ struct ecm_sim {
     int dev = 0;
 
     // agent SoA
-    DBuf<float2> pos[2], vel[2], attraction, pref_vel, v_orca;
+    // slot-indexed state, current when id_current (k_unsort brings it up to
+    // date from the last step's sorted outputs when the host needs it)
+    DBuf<float2> pos, vel, attraction, pref_vel, v_orca;
     DBuf<float> radius, pref_speed;
     DBuf<uint8_t> active;
     DBuf<int> path_off, path_cnt;
@@ -131,10 +135,14 @@ struct ecm_sim {
 
     // agent grid
     DBuf<uint32_t> key, rank, cell_count, cell_start, bsum;
-    DBuf<float4> sorted_pr;
-    DBuf<float2> sorted_v;
+    // sorted state by step parity: this step's [cur], the last step's [cur^1]
+    DBuf<float4> sorted_pr[2], sorted_v[2];
+    DBuf<float2> attr_s[2], pos_s[2], vel_s[2];
+    DBuf<uint8_t> dead_s[2];
+    DBuf<float2> pref_vel_s, v_orca_s;          // this step's only
+    bool sorted_valid = false;                  // the last step's sorted outputs hold the state
+    bool id_current = true;                     // the slot-indexed arrays hold it
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
-    DBuf<int> slot_of[2];              // agent -> sorted position, by step parity
     DBuf<int> fwd, prev_of;            // previous <-> this step's sorted positions
 
     // Per-step counters and the event list share one device block, so one
@@ -221,7 +229,7 @@ struct ecm_sim {
         hipGraphExec_t exec[2] = {nullptr, nullptr};
     };
     bool use_graph = false;
-    StepGraph graphs[2];
+    StepGraph graphs[2][2];                     // [parity][entry from the sorted outputs]
     std::string graph_error;
     std::vector<int> h_replans;
 };
@@ -252,13 +260,17 @@ extern "C" void ecm_sim_reference_params(ecm_sim_params* p) {
 }
 
 static void free_all(ecm_sim* s) {
-    for (int b = 0; b < 2; ++b) { s->pos[b].release(); s->vel[b].release(); }
+    s->pos.release(); s->vel.release();
     s->attraction.release(); s->pref_vel.release(); s->v_orca.release();
+    for (int b = 0; b < 2; ++b) {
+        s->sorted_pr[b].release(); s->sorted_v[b].release(); s->attr_s[b].release();
+        s->pos_s[b].release(); s->vel_s[b].release(); s->dead_s[b].release();
+    }
+    s->pref_vel_s.release(); s->v_orca_s.release();
     s->radius.release(); s->pref_speed.release(); s->active.release();
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
-    s->sorted_pr.release(); s->sorted_v.release();
-    for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); s->slot_of[b].release(); }
+    for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); }
     s->fwd.release(); s->prev_of.release();
     s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
@@ -274,14 +286,15 @@ static void free_all(ecm_sim* s) {
     s->h_stage_cap = s->h_pts_cap = 0;
     s->feat.release();
     s->mlp_mask.release();
-    for (auto& g : s->graphs)
-        for (int h = 0; h < 2; ++h) {
-            if (g.exec[h]) (void)hipGraphExecDestroy(g.exec[h]);
-            if (g.graph[h]) (void)hipGraphDestroy(g.graph[h]);
-            g.exec[h] = nullptr;
-            g.graph[h] = nullptr;
-            g.valid = false;
-        }
+    for (auto& row : s->graphs)
+        for (auto& g : row)
+            for (int h = 0; h < 2; ++h) {
+                if (g.exec[h]) (void)hipGraphExecDestroy(g.exec[h]);
+                if (g.graph[h]) (void)hipGraphDestroy(g.graph[h]);
+                g.exec[h] = nullptr;
+                g.graph[h] = nullptr;
+                g.valid = false;
+            }
     for (auto& e : s->ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (s->ev_stage) (void)hipEventDestroy(s->ev_stage);
     s->ev_stage = nullptr;
@@ -310,6 +323,9 @@ static int settle(ecm_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
+// settle + bring the slot-indexed agent arrays up to date (defined after
+// make_state); for the entry points that read or change agents by slot.
+static int sync_state(ecm_sim* s);
 
 static int zero_dev(void* p, size_t bytes, hipStream_t st) {
     HIPCHK(hipMemsetAsync(p, 0, bytes, st));
@@ -341,10 +357,17 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         delete s;
         return fail(ECM_E_HIP, "stream creation failed");
     }
-    for (int b = 0; b < 2 && !rc; ++b) { rc |= s->pos[b].alloc(N); rc |= s->vel[b].alloc(N); }
+    if (!rc) rc |= s->pos.alloc(N);
+    if (!rc) rc |= s->vel.alloc(N);
     if (!rc) rc |= s->attraction.alloc(N);
     if (!rc) rc |= s->pref_vel.alloc(N);
     if (!rc && p->keep_orca_velocity) rc |= s->v_orca.alloc(N);
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc |= s->sorted_pr[b].alloc(N); rc |= s->sorted_v[b].alloc(N); rc |= s->attr_s[b].alloc(N);
+        rc |= s->pos_s[b].alloc(N); rc |= s->vel_s[b].alloc(N); rc |= s->dead_s[b].alloc(N);
+    }
+    if (!rc) rc |= s->pref_vel_s.alloc(N);
+    if (!rc && p->keep_orca_velocity) rc |= s->v_orca_s.alloc(N);
     if (!rc) rc |= s->radius.alloc(N);
     if (!rc) rc |= s->pref_speed.alloc(N);
     if (!rc) rc |= s->active.alloc(N);
@@ -353,12 +376,9 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->path_pool.alloc(std::max<size_t>(N * 8, 1024));
     if (!rc) rc |= s->key.alloc(N);
     if (!rc) rc |= s->rank.alloc(N);
-    if (!rc) rc |= s->sorted_pr.alloc(N);
-    if (!rc) rc |= s->sorted_v.alloc(N);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc |= s->nbr[b].alloc(N * (size_t)p->k);
         rc |= s->nbr_cnt[b].alloc(N);
-        rc |= s->slot_of[b].alloc(N);
     }
     if (!rc) rc |= s->fwd.alloc(N);
     if (!rc) rc |= s->prev_of.alloc(N);
@@ -377,16 +397,14 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     for (int i = 0; i < 5 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc) {
-        for (int b = 0; b < 2 && !rc; ++b) {
-            rc |= zero_dev(s->pos[b].p, N * sizeof(float2), s->stream);
-            rc |= zero_dev(s->vel[b].p, N * sizeof(float2), s->stream);
-        }
+        rc |= zero_dev(s->pos.p, N * sizeof(float2), s->stream);
+        rc |= zero_dev(s->vel.p, N * sizeof(float2), s->stream);
+        for (int b = 0; b < 2 && !rc; ++b) rc |= zero_dev(s->dead_s[b].p, N, s->stream);
         rc |= zero_dev(s->attraction.p, N * sizeof(float2), s->stream);
         rc |= zero_dev(s->pref_vel.p, N * sizeof(float2), s->stream);
         rc |= zero_dev(s->active.p, N, s->stream);
         rc |= zero_dev(s->path_cnt.p, N * sizeof(int), s->stream);
         for (int b = 0; b < 2; ++b) {      // no seeds before the first step
-            rc |= fill_dev_bytes(s->slot_of[b].p, 0xff, N * sizeof(int), s->stream);
             rc |= zero_dev(s->nbr_cnt[b].p, N * sizeof(int), s->stream);
         }
         rc |= fill_dev_bytes(s->fwd.p, 0xff, N * sizeof(int), s->stream);
@@ -760,7 +778,7 @@ extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_
     if (env < 0 || env >= (int)s->envs.size()) return fail(ECM_E_ARG, "no environment %d", env);
     EnvHost* E = s->envs[env].get();
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     if (n == 0) return 0;
     for (int i = 0; i < n; ++i)
         if (path_cnt[i] < 1 || path_off[i] < 0)
@@ -776,7 +794,7 @@ extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_
         hg.init(E->x0 - 1, E->y0 - 1, E->x1 + 1, E->y1 + 1, gcell);
         if (s->num_agents > 0) {
             cur_pos.resize(s->prm.max_agents);
-            HIPCHK(hipMemcpyAsync(cur_pos.data(), s->pos[s->cur].p, cur_pos.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipMemcpyAsync(cur_pos.data(), s->pos.p, cur_pos.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
             for (int i = 0; i <= s->last_idx; ++i)
                 if (s->h_active[i] && s->h_env[i] == env) hg.add(i, cur_pos[i].x, cur_pos[i].y);
@@ -856,12 +874,13 @@ extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_
         HIPCHK(hipMemcpyAsync(d_st.p, st.data(), n * sizeof(float2), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(d_r.p, radius, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(d_s.p, pref_speed, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
-        HIPCHK(launch_spawn_scatter(n, d_ids.p, d_st.p, d_r.p, d_s.p, d_off.p, d_cnt.p, s->pos[s->cur].p,
-                                    s->vel[s->cur].p, s->radius.p, s->pref_speed.p, s->pref_vel.p, s->active.p,
+        HIPCHK(launch_spawn_scatter(n, d_ids.p, d_st.p, d_r.p, d_s.p, d_off.p, d_cnt.p, s->pos.p,
+                                    s->vel.p, s->radius.p, s->pref_speed.p, s->pref_vel.p, s->active.p,
                                     s->path_off.p, s->path_cnt.p, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     d_ids.release(); d_off.release(); d_cnt.release(); d_st.release(); d_r.release(); d_s.release();
+    s->sorted_valid = false;                    // the next step starts from the slot arrays
     return rc;
 }
 
@@ -883,11 +902,12 @@ extern "C" int ecm_sim_destroy_agent(ecm_sim* s, int id) {
     if (!s || id < 0 || id >= s->prm.max_agents) return fail(ECM_E_ARG, "bad agent id");
     if (!s->h_active[id]) return fail(ECM_E_ARG, "agent %d is not active", id);
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     uint8_t z = 0;
     HIPCHK(hipMemcpyAsync(s->active.p + id, &z, 1, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     mark_destroyed(s, id);
+    s->sorted_valid = false;                    // the next step starts from the slot arrays
     return 0;
 }
 
@@ -1049,21 +1069,26 @@ static DevState make_state(ecm_sim* s) {
     DevState d;
     std::memset(&d, 0, sizeof(d));
     d.n_slots = s->prm.max_agents;
-    d.pos_in = s->pos[s->cur].p; d.pos_out = s->pos[s->cur ^ 1].p;
-    d.vel_in = s->vel[s->cur].p; d.vel_out = s->vel[s->cur ^ 1].p;
+    d.pos_in = s->pos.p;
+    d.vel_in = s->vel.p;
     d.radius = s->radius.p; d.pref_speed = s->pref_speed.p;
-    d.attraction = s->attraction.p; d.pref_vel = s->pref_vel.p; d.active = s->active.p;
+    d.attraction = s->attraction.p; d.active = s->active.p;
     d.path_off = s->path_off.p; d.path_cnt = s->path_cnt.p; d.path_pts = s->path_pool.p;
-    d.v_orca = s->v_orca.p;
+    d.sorted_in = s->sorted_valid ? 1 : 0;
+    const int o = s->cur ^ 1;
+    d.sorted_pr_old = s->sorted_pr[o].p; d.sorted_v_old = s->sorted_v[o].p;
+    d.pos_s_old = s->pos_s[o].p; d.vel_s_old = s->vel_s[o].p;
+    d.attr_s_old = s->attr_s[o].p; d.dead_s_old = s->dead_s[o].p;
+    d.attr_s = s->attr_s[s->cur].p; d.pref_vel_s = s->pref_vel_s.p; d.dead_s = s->dead_s[s->cur].p;
+    d.pos_s = s->pos_s[s->cur].p; d.vel_s = s->vel_s[s->cur].p; d.v_orca_s = s->v_orca_s.p;
     d.n_env = (int)s->envs.size();
     d.envs = s->d_envs.p;
     d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
     d.n_keys = s->n_keys;
     d.key = s->key.p; d.rank = s->rank.p; d.cell_count = s->cell_count.p; d.cell_start = s->cell_start.p;
-    d.sorted_pr = s->sorted_pr.p; d.sorted_v = s->sorted_v.p;
+    d.sorted_pr = s->sorted_pr[s->cur].p; d.sorted_v = s->sorted_v[s->cur].p;
     d.nbr = s->nbr[s->cur].p; d.nbr_cnt = s->nbr_cnt[s->cur].p;
     d.nbr_prev = s->nbr[s->cur ^ 1].p; d.nbr_cnt_prev = s->nbr_cnt[s->cur ^ 1].p;
-    d.slot_of = s->slot_of[s->cur].p; d.slot_of_prev = s->slot_of[s->cur ^ 1].p;
     d.fwd = s->fwd.p; d.prev_of = s->prev_of.p;
     d.events = reinterpret_cast<int4*>(s->xfer.p + 8); d.events_n = s->xfer.p;
     d.stats = s->xfer.p + 2;
@@ -1097,6 +1122,16 @@ static DevState make_state(ecm_sim* s) {
     d.arrival_r2 = s->prm.arrival_radius * s->prm.arrival_radius;
     d.delete_r2 = s->prm.delete_radius * s->prm.delete_radius;
     return d;
+}
+
+static int sync_state(ecm_sim* s) {
+    if (int rc = settle(s)) return rc;
+    if (s->id_current) return 0;
+    const DevState d = make_state(s);        // its "old" arrays are the last step's outputs
+    HIPCHK(launch_unsort(d, s->pos.p, s->vel.p, s->attraction.p, s->pref_vel.p, s->v_orca.p, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->id_current = true;
+    return 0;
 }
 
 // UpdatePath (Simulator.cpp:97-124, 579-587) for the agents whose IRM
@@ -1304,7 +1339,7 @@ static int ensure_step_graph(ecm_sim* s, const DevState& d, int mode) {
     key.frags = mode == 2 ? (const void*)s->mlp->frags.p : nullptr;
     key.pbias = mode == 2 ? (const void*)s->mlp->pbias.p : nullptr;
     key.n_cu = mode == 2 ? s->mlp->n_cu : 0;
-    ecm_sim::StepGraph& g = s->graphs[s->cur];
+    ecm_sim::StepGraph& g = s->graphs[s->cur][d.sorted_in];
     if (g.valid && std::memcmp(&g.key, &key, sizeof(key)) == 0) return 0;
     if (int rc = settle(s)) return rc;      // an earlier launch of g may still run
     drop_graph(g);
@@ -1350,7 +1385,7 @@ static int step_once(ecm_sim* s) {
         else s->use_graph = false;           // capture refused: eager from now on (see graph_error)
     }
     // the whole step; its first half ends by publishing the events
-    if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur].exec[0], s->stream));
+    if (graph) HIPCHK(hipGraphLaunch(s->graphs[s->cur][d.sorted_in].exec[0], s->stream));
     else if ((rc = enqueue_device_half(s, d, mode, 0)) != 0 || (rc = enqueue_device_half(s, d, mode, 1)) != 0)
         return rc;
     const uint32_t peek = (uint32_t)std::min(ecm_sim::kEventPeek, s->prm.max_agents);
@@ -1393,6 +1428,8 @@ static int step_once(ecm_sim* s) {
     // entry point that touches device data waits for the stream first, and
     // the next step is ordered behind this one on the same stream.
     s->cur ^= 1;
+    s->sorted_valid = true;                     // the state now lives in this step's sorted outputs
+    s->id_current = false;
     const int n_before = s->num_agents;
     std::sort(dead.begin(), dead.end());      // reference visits slots in ascending order
     for (int id : dead) mark_destroyed(s, id);
@@ -1530,8 +1567,8 @@ extern "C" int ecm_sim_sync(ecm_sim* s) {
 
 static int component_buf(ecm_sim* s, int c, void** p, size_t* elem) {
     switch (c) {
-        case ECM_C_POSITION: *p = s->pos[s->cur].p; *elem = 8; return 0;
-        case ECM_C_VELOCITY: *p = s->vel[s->cur].p; *elem = 8; return 0;
+        case ECM_C_POSITION: *p = s->pos.p; *elem = 8; return 0;
+        case ECM_C_VELOCITY: *p = s->vel.p; *elem = 8; return 0;
         case ECM_C_PREF_VELOCITY: *p = s->pref_vel.p; *elem = 8; return 0;
         case ECM_C_ATTRACTION: *p = s->attraction.p; *elem = 8; return 0;
         case ECM_C_CLEARANCE: *p = s->radius.p; *elem = 4; return 0;
@@ -1552,7 +1589,7 @@ extern "C" int ecm_sim_read(ecm_sim* s, int c, void* dst, size_t bytes) {
     if (rc) return rc;
     if (bytes != el * (size_t)s->prm.max_agents) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * (size_t)s->prm.max_agents);
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
@@ -1567,7 +1604,7 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
     const size_t N = (size_t)s->prm.max_agents;
     if (bytes != el * N) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * N);
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     const float* f = (const float*)src;
@@ -1587,19 +1624,14 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
             else s->free_stack.push_back(i);
         }
     }
-    if (c == ECM_C_POSITION || c == ECM_C_VELOCITY) {
-        // keep both buffers equal so inactive slots read back consistently
-        void* other = (c == ECM_C_POSITION) ? (void*)s->pos[s->cur ^ 1].p : (void*)s->vel[s->cur ^ 1].p;
-        HIPCHK(hipMemcpyAsync(other, src, bytes, hipMemcpyHostToDevice, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
-    }
+    s->sorted_valid = false;                    // the next step starts from the slot arrays
     return 0;
 }
 
 extern "C" int ecm_sim_device_ptr(ecm_sim* s, int c, void** out) {
     if (!s || !out) return fail(ECM_E_ARG, "null argument");
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     size_t el;
     return component_buf(s, c, out, &el);
 }
@@ -1631,6 +1663,7 @@ extern "C" int ecm_sim_get_neighbors(ecm_sim* s, int* nbr, int* cnt) {
     DevState d = make_state(s);
     d.nbr = s->nbr[s->cur ^ 1].p;
     d.nbr_cnt = s->nbr_cnt[s->cur ^ 1].p;
+    d.sorted_pr = s->sorted_pr[s->cur ^ 1].p;
     HIPCHK(launch_export_nbrs(d, dn.p, dc.p, s->stream));
     HIPCHK(hipMemcpyAsync(nbr, dn.p, N * s->prm.k * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(cnt, dc.p, N * sizeof(int), hipMemcpyDeviceToHost, s->stream));
@@ -1764,7 +1797,7 @@ static bool rd(FILE* f, T* p, size_t n) { return fread(p, sizeof(T), n, f) == n;
 extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
     if (!s || !path) return fail(ECM_E_ARG, "null argument");
     (void)hipSetDevice(s->dev);
-    if (int rc_ = settle(s)) return rc_;
+    if (int rc_ = sync_state(s)) return rc_;
     const size_t N = (size_t)s->prm.max_agents;
     std::vector<float2> pos(N), vel(N), attr(N), pv(N);
     std::vector<float> rad(N), spd(N);

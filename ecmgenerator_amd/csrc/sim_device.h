@@ -50,21 +50,44 @@ struct EnvDesc {
 
 // Everything the step kernels read or write.  Agent arrays are indexed by
 // slot id (entity == slot, Simulator.cpp:40); "sorted" arrays by the
-// position of the agent in the Morton-ordered grid.
+// position of the agent in this step's Morton-ordered grid.
+//
+// Between steps the state lives in the sorted outputs of the last step
+// (positions, velocities, attraction points by its sorted position); the
+// slot-indexed arrays are brought up to date only when the host asks
+// (k_unsort).  A step builds its grid from those sorted outputs (the order
+// barely changes, so every access is close to coalesced), or from the
+// slot-indexed arrays after the host changed them (spawn, destroy, write,
+// snapshot load).
 struct DevState {
-    // ---- agent components (Simulator.h:179-187) ----
+    // ---- agent components by slot (Simulator.h:179-187) ----
     int n_slots;
-    const float2* pos_in;   float2* pos_out;     // double-buffered positions
-    const float2* vel_in;   float2* vel_out;     // double-buffered velocities
+    const float2* pos_in;                        // positions
+    const float2* vel_in;                        // velocities
     const float* radius;                         // ClearanceComponent
     const float* pref_speed;                     // SpeedComponent
-    float2* attraction;                          // attraction point (persistent)
-    float2* pref_vel;                            // preferred velocity
+    const float2* attraction;                    // attraction points
     uint8_t* active;                             // active flags
     const int* path_off;                         // PathComponent as CSR
     const int* path_cnt;
     const float2* path_pts;
-    float2* v_orca;                              // ORCA output velocity (diagnostic, may be null)
+    int sorted_in;                               // 1: grid from the last step's sorted outputs
+
+    // ---- the last step's sorted outputs (read when sorted_in) ----
+    const float4* sorted_pr_old;   // (x, y, radius, id bits) of its step start
+    const float4* sorted_v_old;    // (vx, vy, preferred speed, environment bits)
+    const float2* pos_s_old;       // its end-of-step positions
+    const float2* vel_s_old;       // its end-of-step velocities
+    const float2* attr_s_old;      // its attraction points
+    const uint8_t* dead_s_old;     // destroyed during it (its active count is acc[13])
+
+    // ---- this step's state by sorted position ----
+    float2* attr_s;                // attraction point (carried in, updated by k_attr_knn)
+    float2* pref_vel_s;            // preferred velocity
+    uint8_t* dead_s;               // destroyed this step
+    float2* pos_s;                 // end-of-step position
+    float2* vel_s;                 // end-of-step velocity
+    float2* v_orca_s;              // ORCA output velocity (diagnostic, may be null)
 
     // ---- environments ----
     int n_env;
@@ -78,13 +101,11 @@ struct DevState {
     uint32_t* cell_count;   // [n_keys]   zero between steps (the scan clears it)
     uint32_t* cell_start;   // [n_keys + 1] exclusive scan; [n_keys] = active count
     float4* sorted_pr;      // [n_slots] (x, y, radius, id bits)
-    float2* sorted_v;       // [n_slots] (vx, vy)
+    float4* sorted_v;       // [n_slots] (vx, vy, preferred speed, environment bits)
     int* nbr;               // [k][n_slots] neighbour sorted positions (SoA)
     int* nbr_cnt;           // [n_slots]   by sorted position
     // neighbour-search seeds: each agent's neighbours of the previous step
     // bound its k-th distance before the search starts
-    int* slot_of;             // [n_slots] sorted position of each agent, -1 off the grid
-    const int* slot_of_prev;  // the same map of the previous step
     const int* nbr_prev;      // the previous step's nbr and nbr_cnt (by its sorted positions)
     const int* nbr_cnt_prev;
     int* fwd;                 // [n_slots] previous sorted position -> this step's, -1: left the grid
@@ -99,7 +120,8 @@ struct DevState {
                           // length, [2] locate fallbacks to the exact-y list, [3..5] unused
     uint32_t* wstats;     // per step, [64][4] slots: learned proposals tried, accepted,
                           // constraints proposed, unused
-    unsigned long long* acc;   // [16] totals of the finished steps; [12] steps started
+    unsigned long long* acc;   // [16] totals of the finished steps; [12] steps started,
+                               // [13] the last grid's active count, [14] the previous one
     // host-mapped [1 + 8 + 4 * event_peek] words: [0] number of the last
     // step that published, then its counters and first events
     uint32_t* host_xfer;

@@ -37,8 +37,21 @@ __device__ __forceinline__ uint32_t part1by1(uint32_t x) {
 __device__ __forceinline__ uint32_t morton2(uint32_t cx, uint32_t cy) {
     return part1by1(cx) | (part1by1(cy) << 1);
 }
-__device__ __forceinline__ const EnvDesc& env_of(const DevState& s, int id) {
-    return s.envs[s.agent_env ? s.agent_env[id] : 0];
+__device__ __forceinline__ int slot_env(const DevState& s, int id) {
+    return s.agent_env ? (int)s.agent_env[id] : 0;
+}
+// Velocity of a sorted agent (the first half of its sorted_v entry).
+__device__ __forceinline__ float2 sorted_vel(const DevState& s, int t) {
+    return *reinterpret_cast<const float2*>(&s.sorted_v[t]);
+}
+// Environment of a sorted agent from its sorted_v entry (w = environment bits).
+__device__ __forceinline__ const EnvDesc& env_of_sorted(const DevState& s, float4 v) {
+    return s.envs[__float_as_int(v.w)];
+}
+__device__ __forceinline__ uint32_t cell_key(const EnvDesc& E, float2 p) {
+    const int cx = grid_cell(p.x, E.ag.x0, E.ag.inv_cell, E.ag.side);
+    const int cy = grid_cell(p.y, E.ag.y0, E.ag.inv_cell, E.ag.side);
+    return E.cell_base + morton2((uint32_t)cx, (uint32_t)cy);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i -> XCD i % 8).
@@ -70,24 +83,36 @@ __device__ __forceinline__ void step_begin(uint32_t* counters, uint32_t* wstats,
     wstats[t] = 0u;
 }
 
+// Histogram pass, entered from the slot-indexed arrays: one element per
+// slot id; the atomic's return value is the agent's arrival order in its cell.
 __global__ __launch_bounds__(256) void k_grid_count(DevState s) {
     if (blockIdx.x == 0) step_begin(s.events_n, s.wstats, s.acc);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
-    if (!s.active[i]) {
-        s.key[i] = kInvalidKey;
-        s.slot_of[i] = -1;
-        const int q = s.slot_of_prev[i];
-        if (q >= 0) s.fwd[q] = -1;           // on the grid last step, gone now
-        return;
-    }
-    float2 p = s.pos_in[i];
-    const EnvDesc& E = env_of(s, i);
-    int cx = grid_cell(p.x, E.ag.x0, E.ag.inv_cell, E.ag.side);
-    int cy = grid_cell(p.y, E.ag.y0, E.ag.inv_cell, E.ag.side);
-    uint32_t m = E.cell_base + morton2((uint32_t)cx, (uint32_t)cy);
+    if (!s.active[i]) { s.key[i] = kInvalidKey; return; }
+    const uint32_t m = cell_key(s.envs[slot_env(s, i)], s.pos_in[i]);
     s.key[i] = m;
     s.rank[i] = atomicAdd(&s.cell_count[m], 1u);
+}
+
+// Histogram pass, entered from the last step's sorted outputs: one element
+// per previous sorted position (in the previous Morton order, so the cells
+// of a wavefront are few and close); agents destroyed during the last step
+// leave the grid.
+// The last step's active count is acc[13] (its scan); it is kept in
+// acc[14] for the scatter, since this step's scan overwrites acc[13].
+__global__ __launch_bounds__(256) void k_grid_count_sorted(DevState s) {
+    const uint32_t n_old = (uint32_t)s.acc[13];
+    if (blockIdx.x == 0) {
+        step_begin(s.events_n, s.wstats, s.acc);
+        if (threadIdx.x == 0) s.acc[14] = n_old;
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)n_old) return;
+    if (s.dead_s_old[t]) { s.key[t] = kInvalidKey; s.fwd[t] = -1; return; }
+    const uint32_t m = cell_key(env_of_sorted(s, s.sorted_v_old[t]), s.pos_s_old[t]);
+    s.key[t] = m;
+    s.rank[t] = atomicAdd(&s.cell_count[m], 1u);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -133,7 +158,8 @@ __global__ __launch_bounds__(256) void k_scan_local(uint32_t* cnt, uint32_t* sta
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 // single block: exclusive scan of the block sums, grand total to start[n]
-__global__ __launch_bounds__(256) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* start, uint32_t n) {
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* start, uint32_t n,
+                                                  unsigned long long* n_active) {
     uint32_t carry = 0;
     for (uint32_t c = 0; c < nb; c += 1024u) {
         uint32_t base = c + threadIdx.x * 4u;
@@ -150,38 +176,63 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* bsum, uint32_t nb, u
         }
         carry += tot;
     }
-    if (threadIdx.x == 0) start[n] = carry;
+    if (threadIdx.x == 0) { start[n] = carry; *n_active = carry; }
 }
 __global__ __launch_bounds__(256) void k_scan_add(uint32_t* start, const uint32_t* bsum, uint32_t n) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) start[i] += bsum[i >> 10];
 }
 
-// Counting-sort scatter: the slot is the cell's start plus the arrival
-// order the count pass recorded.
+// Counting-sort scatter from the slot-indexed arrays: the sorted position
+// is the cell's start plus the arrival order the count pass recorded.  No
+// neighbour seeds after the host changed the agents (prev_of = -1).
 __global__ void k_grid_scatter(DevState s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
     uint32_t m = s.key[i];
     if (m == kInvalidKey) return;
     uint32_t slot = s.cell_start[m] + s.rank[i];
-    s.slot_of[i] = (int)slot;
-    float2 p = s.pos_in[i];
+    const float2 p = s.pos_in[i], v = s.vel_in[i];
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
-    s.sorted_v[slot] = s.vel_in[i];
+    s.sorted_v[slot] = make_float4(v.x, v.y, s.pref_speed[i], __int_as_float(slot_env(s, i)));
+    s.attr_s[slot] = s.attraction[i];
+    s.prev_of[slot] = -1;
 }
 
-// Maps between the previous and this step's sorted positions, in sorted
-// order: the agent order barely changes from step to step, so the writes
-// to fwd land close together (agents that left the grid were cleared by
-// k_grid_count).
-__global__ void k_seed_map(DevState s) {
-    const uint32_t n_active = s.cell_start[s.n_keys];
+// Counting-sort scatter from the last step's sorted outputs: reads and
+// writes both follow the Morton order, which barely changes from step to
+// step.  Also records the maps between the two orders for the neighbour
+// seeds.
+__global__ void k_grid_scatter_sorted(DevState s) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int)n_active) return;
-    const int q = s.slot_of_prev[__float_as_int(s.sorted_pr[t].w)];
-    s.prev_of[t] = q;
-    if (q >= 0) s.fwd[q] = t;
+    if (t >= (int)(uint32_t)s.acc[14]) return;
+    const uint32_t m = s.key[t];
+    if (m == kInvalidKey) return;
+    const uint32_t slot = s.cell_start[m] + s.rank[t];
+    const float2 p = s.pos_s_old[t], v = s.vel_s_old[t];
+    const float4 pr = s.sorted_pr_old[t], sv = s.sorted_v_old[t];
+    s.sorted_pr[slot] = make_float4(p.x, p.y, pr.z, pr.w);
+    s.sorted_v[slot] = make_float4(v.x, v.y, sv.z, sv.w);
+    s.attr_s[slot] = s.attr_s_old[t];
+    s.fwd[t] = (int)slot;
+    s.prev_of[slot] = t;
+}
+
+// Brings the slot-indexed arrays up to date from the last step's sorted
+// outputs (the host asks before it reads or changes agents).  Agents
+// destroyed during that step keep their slot's preferred and ORCA
+// velocities, as the reference leaves them.
+__global__ void k_unsort(DevState s, float2* pos, float2* vel, float2* attraction, float2* pref_vel, float2* v_orca) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)(uint32_t)s.acc[13]) return;         // the last step's active count
+    const int id = __float_as_int(s.sorted_pr_old[t].w);
+    pos[id] = s.pos_s_old[t];
+    vel[id] = s.vel_s_old[t];
+    attraction[id] = s.attr_s_old[t];
+    if (!s.dead_s_old[t]) {
+        pref_vel[id] = s.pref_vel_s[t];
+        if (v_orca) v_orca[id] = s.v_orca_s[t];
+    }
 }
 
 // ------------------------------------------------------------- ECM query
@@ -367,7 +418,7 @@ template <int K>   // K == k
 __device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id, V2 p) {
     const int q = s.prev_of[t];
     if (q < 0 || q >= s.n_slots || s.nbr_cnt_prev[q] < K) return INFINITY;
-    const int env = s.agent_env ? (int)s.agent_env[id] : 0;
+    const int env = s.agent_env ? __float_as_int(s.sorted_v[t].w) : 0;
     int ns[K];
     bool ok = true;
 #pragma unroll
@@ -387,7 +438,7 @@ __device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id
         const float dx = c.x - p.x, dy = c.y - p.y;
         const float d2 = dx * dx + dy * dy;
         ok &= d2 > kEps;
-        if (s.agent_env) ok &= (int)s.agent_env[__float_as_int(c.w)] == env;
+        if (s.agent_env) ok &= __float_as_int(s.sorted_v[ns[j]].w) == env;
         bound = fmaxf(bound, d2);
     }
     return ok ? bound : INFINITY;
@@ -507,9 +558,10 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return false;
     float4 pr = s.sorted_pr[t];
+    const float4 sv = s.sorted_v[t];
     const int id = __float_as_int(pr.w);
     const V2 pos = mk(pr.x, pr.y);
-    const EnvDesc& E = env_of(s, id);
+    const EnvDesc& E = env_of_sorted(s, sv);
 
     // UpdateAttractionPointSystem (Simulator.cpp:538-590)
     const int off = s.path_off[id], np = s.path_cnt[id];
@@ -517,7 +569,7 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     float2 gg = path[np - 1];
     float gx = pos.x - gg.x, gy = pos.y - gg.y;
     float dGoal = gx * gx + gy * gy;
-    float2 a0 = s.attraction[id];
+    float2 a0 = s.attr_s[t];
     V2 attr = mk(a0.x, a0.y);
     bool alive = true, event = false;
     uint32_t fb = 0;
@@ -534,8 +586,9 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
             event = true;
         }
     }
-    s.attraction[id] = make_float2(attr.x, attr.y);
+    s.attr_s[t] = make_float2(attr.x, attr.y);
     if (fb) atomicAdd(&s.stats[2], 1u);
+    s.dead_s[t] = alive ? 0 : 1;
     if (!alive) {
         s.active[id] = 0;                                   // DestroyAgent
         uint32_t q = atomicAdd(s.events_n, 1u);
@@ -545,9 +598,9 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
         return true;
     }
     // ApplySteeringForce (Simulator.cpp:638-657)
-    const float speed = s.pref_speed[id];
+    const float speed = sv.z;
     V2 des = normalize(attr - pos) * speed;
-    s.pref_vel[id] = make_float2(des.x, des.y);
+    s.pref_vel_s[t] = make_float2(des.x, des.y);
 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
@@ -845,13 +898,14 @@ __device__ void lp3d(float4* lds, int tid, int n, int nObst, int proj_base, floa
 
 // Velocity + position systems and the force between them
 // (Simulator.cpp:592-635, 676-677).
-__device__ __forceinline__ void integrate(const DevState& s, int id, V2 position, V2 velocity, V2 o) {
+// Written by sorted position t.
+__device__ __forceinline__ void integrate(const DevState& s, int t, V2 position, V2 velocity, V2 o) {
     const V2 F = mk(o.x - velocity.x, o.y - velocity.y);
     const V2 nv = mk(velocity.x + F.x * s.mass_recip * s.dt, velocity.y + F.y * s.mass_recip * s.dt);
     const V2 npn = mk(position.x + (nv.x * s.dt), position.y + (nv.y * s.dt));
-    s.pos_out[id] = make_float2(npn.x, npn.y);
-    s.vel_out[id] = make_float2(nv.x, nv.y);
-    if (s.v_orca) s.v_orca[id] = make_float2(o.x, o.y);
+    s.pos_s[t] = make_float2(npn.x, npn.y);
+    s.vel_s[t] = make_float2(nv.x, nv.y);
+    if (s.v_orca_s) s.v_orca_s[t] = make_float2(o.x, o.y);
 }
 
 // ----------------------------------------------- learned-proposal support
@@ -903,7 +957,7 @@ __device__ void agent_features(const DevState& s, const EnvDesc& E, int t, V2 po
         if (j < cnt) {
             const int sj = s.nbr[(size_t)j * s.n_slots + t];
             const float4 q = s.sorted_pr[sj];
-            const float2 qv = s.sorted_v[sj];
+            const float2 qv = sorted_vel(s, sj);
             f[4 * j + 0] = feat_clamp((q.x - position.x) * mlp::kPosScale);
             f[4 * j + 1] = feat_clamp((q.y - position.y) * mlp::kPosScale);
             f[4 * j + 2] = feat_clamp((qv.x - velocity.x) * mlp::kVelScale);
@@ -1040,20 +1094,21 @@ constexpr float kActiveTol = 1e-3f;   // a constraint is labelled active within 
 template <int K, int OSLOTS, bool OVERFLOW, int MODE>
 __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds, int tid) {
     const float4 pr = s.sorted_pr[t];
-    const float2 v0 = s.sorted_v[t];
+    const float4 sv = s.sorted_v[t];
+    const float2 v0 = make_float2(sv.x, sv.y);
     const int id = __float_as_int(pr.w);
-    if (!s.active[id]) {                 // destroyed this step: state unchanged
-        s.pos_out[id] = make_float2(pr.x, pr.y);
-        s.vel_out[id] = v0;
+    if (s.dead_s[t]) {                   // destroyed this step: state unchanged
+        s.pos_s[t] = make_float2(pr.x, pr.y);
+        s.vel_s[t] = v0;
         return;
     }
     const V2 position = mk(pr.x, pr.y);
     const V2 velocity = mk(v0.x, v0.y);
     const float rad = pr.z;
-    const float speed = s.pref_speed[id];
+    const float speed = sv.z;
     const float maxSpeed = s.max_speed_from_pref ? speed : s.max_speed;
-    const float2 pv = s.pref_vel[id];
-    const EnvDesc& E = env_of(s, id);
+    const float2 pv = s.pref_vel_s[t];
+    const EnvDesc& E = env_of_sorted(s, sv);
 
     // obstacle neighbours in scan order, constraints generated in place
     int nc = 0;
@@ -1084,7 +1139,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     for (int j = 0; j < cnt; ++j) {
         int sj = s.nbr[(size_t)j * s.n_slots + t];
         float4 q = s.sorted_pr[sj];
-        float2 qv = s.sorted_v[sj];
+        float2 qv = sorted_vel(s, sj);
         V2 cp, cn;
         agent_constraint(s, position, velocity, rad, q, qv, cp, cn);
         con_st(lds, nc, tid, cp, cn);
@@ -1120,7 +1175,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
             for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
             return;
         }
-        integrate(s, id, position, velocity, o);
+        integrate(s, t, position, velocity, o);
         return;
     }
 
@@ -1165,7 +1220,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
             s.log_v[idx] = make_float2(o.x, o.y);
         }
     }
-    integrate(s, id, position, velocity, o);
+    integrate(s, t, position, velocity, o);
 }
 
 // Main ORCA pass: one agent per lane in Morton order, a small constraint
@@ -1211,19 +1266,19 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
         const float4* src = s.q3d_con + (size_t)q * s.con_stride;
         for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
         const float4 pr = s.sorted_pr[t];
-        const float2 v0 = s.sorted_v[t];
-        const int id = __float_as_int(pr.w);
-        const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
+        const float4 sv = s.sorted_v[t];
+        const float2 v0 = make_float2(sv.x, sv.y);
+        const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
         V2 o;
         if (info.z < 0) {
-            const float2 pv = s.pref_vel[id];
+            const float2 pv = s.pref_vel_s[t];
             info.z = lp2d(lds, tid, info.x, info.x, 0, mk(pv.x, pv.y), maxSpeed, false, o);
         } else {
             const float2 ov = s.q3d_vel[q];
             o = mk(ov.x, ov.y);
         }
         if (info.z < info.x) lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
-        integrate(s, id, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
+        integrate(s, t, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
     }
 }
 
@@ -1235,10 +1290,10 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return;
     const float4 pr = s.sorted_pr[t];
-    const float2 v0 = s.sorted_v[t];
-    const int id = __float_as_int(pr.w);
+    const float4 sv = s.sorted_v[t];
+    const float2 v0 = make_float2(sv.x, sv.y);
     uint4* out = s.feat + (size_t)t * 8;
-    if (!s.active[id]) {
+    if (s.dead_s[t]) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) out[c] = make_uint4(0, 0, 0, 0);
         return;
@@ -1246,8 +1301,8 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
     const V2 position = mk(pr.x, pr.y);
     const V2 velocity = mk(v0.x, v0.y);
     const float rad = pr.z;
-    const float maxSpeed = s.max_speed_from_pref ? s.pref_speed[id] : s.max_speed;
-    const EnvDesc& E = env_of(s, id);
+    const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
+    const EnvDesc& E = env_of_sorted(s, sv);
     Near4 T;
     near4_init(T);
     if (E.n_obst_edges > 0) {
@@ -1257,7 +1312,7 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
             return true;
         });
     }
-    agent_features<K>(s, E, t, position, velocity, rad, s.pref_vel[id], s.nbr_cnt[t], T, out);
+    agent_features<K>(s, E, t, position, velocity, rad, s.pref_vel_s[t], s.nbr_cnt[t], T, out);
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
@@ -1276,15 +1331,28 @@ __global__ void k_export_nbrs(DevState s, int* out, int* out_cnt) {
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 static inline int round8(int n) { return (n + 7) & ~7; }
 
+// From the last step's sorted outputs (s.sorted_in) the count and scatter
+// passes cover its agents (the count is read on the device); from the slot
+// arrays, every slot.  The launch shapes depend only on the capacity, so
+// one captured graph serves each entry for any agent count.
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) {
     const uint32_t G = s.n_keys;
-    hipLaunchKernelGGL(k_grid_count, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
+    const dim3 ga(cdiv(s.n_slots, 256)), b(256);
+    if (s.sorted_in) hipLaunchKernelGGL(k_grid_count_sorted, ga, b, 0, st, s);
+    else hipLaunchKernelGGL(k_grid_count, ga, b, 0, st, s);
     const uint32_t nb = (G + 1023u) / 1024u;
     hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, st, s.cell_count, s.cell_start, bsum, G);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb, s.cell_start, G);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb, s.cell_start, G, s.acc + 13);
     hipLaunchKernelGGL(k_scan_add, dim3(cdiv(G, 256)), dim3(256), 0, st, s.cell_start, (const uint32_t*)bsum, G);
-    hipLaunchKernelGGL(k_grid_scatter, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_seed_map, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
+    if (s.sorted_in) hipLaunchKernelGGL(k_grid_scatter_sorted, ga, b, 0, st, s);
+    else hipLaunchKernelGGL(k_grid_scatter, ga, b, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_unsort(const DevState& s, float2* pos, float2* vel, float2* attraction, float2* pref_vel,
+                         float2* v_orca, hipStream_t st) {
+    hipLaunchKernelGGL(k_unsort, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s, pos, vel, attraction, pref_vel,
+                       v_orca);
     return hipGetLastError();
 }
 
