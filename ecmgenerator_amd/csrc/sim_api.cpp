@@ -23,8 +23,7 @@
 
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
-hipError_t launch_unsort(const DevState& s, float2* pos, float2* vel, float2* attraction, float2* pref_vel,
-                         float2* v_orca, hipStream_t st);
+hipError_t launch_unsort(const DevState& s, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
@@ -139,7 +138,7 @@ struct ecm_sim {
     DBuf<float4> sorted_pr[2], sorted_v[2];
     DBuf<float2> attr_s[2], pos_s[2], vel_s[2];
     DBuf<uint8_t> dead_s[2];
-    DBuf<float2> pref_vel_s, v_orca_s;          // this step's only
+    DBuf<float2> pref_vel_s[2], v_orca_s[2];
     bool sorted_valid = false;                  // the last step's sorted outputs hold the state
     bool id_current = true;                     // the slot-indexed arrays hold it
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
@@ -266,7 +265,7 @@ static void free_all(ecm_sim* s) {
         s->sorted_pr[b].release(); s->sorted_v[b].release(); s->attr_s[b].release();
         s->pos_s[b].release(); s->vel_s[b].release(); s->dead_s[b].release();
     }
-    s->pref_vel_s.release(); s->v_orca_s.release();
+    for (int b = 0; b < 2; ++b) { s->pref_vel_s[b].release(); s->v_orca_s[b].release(); }
     s->radius.release(); s->pref_speed.release(); s->active.release();
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
@@ -366,8 +365,10 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc |= s->sorted_pr[b].alloc(N); rc |= s->sorted_v[b].alloc(N); rc |= s->attr_s[b].alloc(N);
         rc |= s->pos_s[b].alloc(N); rc |= s->vel_s[b].alloc(N); rc |= s->dead_s[b].alloc(N);
     }
-    if (!rc) rc |= s->pref_vel_s.alloc(N);
-    if (!rc && p->keep_orca_velocity) rc |= s->v_orca_s.alloc(N);
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc |= s->pref_vel_s[b].alloc(N);
+        if (p->keep_orca_velocity) rc |= s->v_orca_s[b].alloc(N);
+    }
     if (!rc) rc |= s->radius.alloc(N);
     if (!rc) rc |= s->pref_speed.alloc(N);
     if (!rc) rc |= s->active.alloc(N);
@@ -1072,15 +1073,16 @@ static DevState make_state(ecm_sim* s) {
     d.pos_in = s->pos.p;
     d.vel_in = s->vel.p;
     d.radius = s->radius.p; d.pref_speed = s->pref_speed.p;
-    d.attraction = s->attraction.p; d.active = s->active.p;
+    d.attraction = s->attraction.p; d.pref_vel = s->pref_vel.p; d.v_orca = s->v_orca.p; d.active = s->active.p;
     d.path_off = s->path_off.p; d.path_cnt = s->path_cnt.p; d.path_pts = s->path_pool.p;
     d.sorted_in = s->sorted_valid ? 1 : 0;
     const int o = s->cur ^ 1;
     d.sorted_pr_old = s->sorted_pr[o].p; d.sorted_v_old = s->sorted_v[o].p;
     d.pos_s_old = s->pos_s[o].p; d.vel_s_old = s->vel_s[o].p;
     d.attr_s_old = s->attr_s[o].p; d.dead_s_old = s->dead_s[o].p;
-    d.attr_s = s->attr_s[s->cur].p; d.pref_vel_s = s->pref_vel_s.p; d.dead_s = s->dead_s[s->cur].p;
-    d.pos_s = s->pos_s[s->cur].p; d.vel_s = s->vel_s[s->cur].p; d.v_orca_s = s->v_orca_s.p;
+    d.pref_vel_s_old = s->pref_vel_s[o].p; d.v_orca_s_old = s->v_orca_s[o].p;
+    d.attr_s = s->attr_s[s->cur].p; d.pref_vel_s = s->pref_vel_s[s->cur].p; d.dead_s = s->dead_s[s->cur].p;
+    d.pos_s = s->pos_s[s->cur].p; d.vel_s = s->vel_s[s->cur].p; d.v_orca_s = s->v_orca_s[s->cur].p;
     d.n_env = (int)s->envs.size();
     d.envs = s->d_envs.p;
     d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
@@ -1128,7 +1130,7 @@ static int sync_state(ecm_sim* s) {
     if (int rc = settle(s)) return rc;
     if (s->id_current) return 0;
     const DevState d = make_state(s);        // its "old" arrays are the last step's outputs
-    HIPCHK(launch_unsort(d, s->pos.p, s->vel.p, s->attraction.p, s->pref_vel.p, s->v_orca.p, s->stream));
+    HIPCHK(launch_unsort(d, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->id_current = true;
     return 0;

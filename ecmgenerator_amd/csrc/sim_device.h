@@ -62,11 +62,13 @@ struct EnvDesc {
 struct DevState {
     // ---- agent components by slot (Simulator.h:179-187) ----
     int n_slots;
-    const float2* pos_in;                        // positions
-    const float2* vel_in;                        // velocities
+    float2* pos_in;                              // positions
+    float2* vel_in;                              // velocities
     const float* radius;                         // ClearanceComponent
     const float* pref_speed;                     // SpeedComponent
-    const float2* attraction;                    // attraction points
+    float2* attraction;                          // attraction points
+    float2* pref_vel;                            // preferred velocities
+    float2* v_orca;                              // ORCA output velocities (diagnostic, may be null)
     uint8_t* active;                             // active flags
     const int* path_off;                         // PathComponent as CSR
     const int* path_cnt;
@@ -80,6 +82,8 @@ struct DevState {
     const float2* vel_s_old;       // its end-of-step velocities
     const float2* attr_s_old;      // its attraction points
     const uint8_t* dead_s_old;     // destroyed during it (its active count is acc[13])
+    const float2* pref_vel_s_old;  // its preferred velocities
+    const float2* v_orca_s_old;    // its ORCA velocities (may be null)
 
     // ---- this step's state by sorted position ----
     float2* attr_s;                // attraction point (carried in, updated by k_attr_knn)

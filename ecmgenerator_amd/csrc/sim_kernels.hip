@@ -220,19 +220,17 @@ __global__ void k_grid_scatter_sorted(DevState s) {
 
 // Brings the slot-indexed arrays up to date from the last step's sorted
 // outputs (the host asks before it reads or changes agents).  Agents
-// destroyed during that step keep their slot's preferred and ORCA
-// velocities, as the reference leaves them.
-__global__ void k_unsort(DevState s, float2* pos, float2* vel, float2* attraction, float2* pref_vel, float2* v_orca) {
+// destroyed during a step had their slots written then (k_attr_knn).
+__global__ void k_unsort(DevState s) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int)(uint32_t)s.acc[13]) return;         // the last step's active count
+    if (s.dead_s_old[t]) return;                       // written when it was destroyed
     const int id = __float_as_int(s.sorted_pr_old[t].w);
-    pos[id] = s.pos_s_old[t];
-    vel[id] = s.vel_s_old[t];
-    attraction[id] = s.attr_s_old[t];
-    if (!s.dead_s_old[t]) {
-        pref_vel[id] = s.pref_vel_s[t];
-        if (v_orca) v_orca[id] = s.v_orca_s[t];
-    }
+    s.pos_in[id] = s.pos_s_old[t];
+    s.vel_in[id] = s.vel_s_old[t];
+    s.attraction[id] = s.attr_s_old[t];
+    s.pref_vel[id] = s.pref_vel_s_old[t];
+    if (s.v_orca) s.v_orca[id] = s.v_orca_s_old[t];
 }
 
 // ------------------------------------------------------------- ECM query
@@ -590,6 +588,18 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     if (fb) atomicAdd(&s.stats[2], 1u);
     s.dead_s[t] = alive ? 0 : 1;
     if (!alive) {
+        // the slot keeps the state it is destroyed with: step-start position
+        // and velocity, this step's attraction point, the previous step's
+        // preferred and ORCA velocities (already there after a step that
+        // started from the slot arrays)
+        s.pos_in[id] = make_float2(pos.x, pos.y);
+        s.vel_in[id] = make_float2(sv.x, sv.y);
+        s.attraction[id] = make_float2(attr.x, attr.y);
+        const int qp = s.prev_of[t];
+        if (qp >= 0) {
+            s.pref_vel[id] = s.pref_vel_s_old[qp];
+            if (s.v_orca) s.v_orca[id] = s.v_orca_s_old[qp];
+        }
         s.active[id] = 0;                                   // DestroyAgent
         uint32_t q = atomicAdd(s.events_n, 1u);
         s.events[q] = make_int4(id, 0, 0, 1);
@@ -1349,10 +1359,8 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
     return hipGetLastError();
 }
 
-hipError_t launch_unsort(const DevState& s, float2* pos, float2* vel, float2* attraction, float2* pref_vel,
-                         float2* v_orca, hipStream_t st) {
-    hipLaunchKernelGGL(k_unsort, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s, pos, vel, attraction, pref_vel,
-                       v_orca);
+hipError_t launch_unsort(const DevState& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_unsort, dim3(cdiv(s.n_slots, 256)), dim3(256), 0, st, s);
     return hipGetLastError();
 }
 
