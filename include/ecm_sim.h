@@ -167,7 +167,11 @@ int ecm_sim_sync(ecm_sim* s);
 int ecm_sim_read(ecm_sim* s, int component, void* dst, size_t bytes);
 /* Overwrite a component (snapshot restore / state injection). */
 int ecm_sim_write(ecm_sim* s, int component, const void* src, size_t bytes);
-/* Device address of a component of the current step's input state. */
+/* Device address of a component, up to date at the call (between steps
+ * the state lives in the last step's sorted arrays; the slot-indexed
+ * component arrays are refreshed when the host asks).  The array is not
+ * updated by later steps until the next read / device_ptr / sync call, and
+ * writing through it does not change the simulation: use ecm_sim_write. */
 int ecm_sim_device_ptr(ecm_sim* s, int component, void** out);
 
 int ecm_sim_num_agents(ecm_sim* s, int* out);    /* GetNumAgents (Simulator.h:114) */
