@@ -442,6 +442,44 @@ __device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id
     return ok ? bound : INFINITY;
 }
 
+// Merge the nb buffered candidates (sorted positions, entry e at
+// buf[e * bstride]) into the ordered list (d, sl) holding n entries, and
+// tighten the keep threshold thr (and the squared cell-pruning radius
+// prune2) once the list holds k.
+template <int K>
+__device__ __forceinline__ void knn_merge(const DevState& s, V2 p, const int* buf, int bstride, int& nb,
+                                          float (&d)[K], int (&sl)[K], int& n, float& thr, float& prune2) {
+    for (int e = 0; e < nb; ++e) {
+        const int j = buf[e * bstride];
+        const float4 c = s.sorted_pr[j];
+        const float dx = c.x - p.x, dy = c.y - p.y;
+        const float d2 = dx * dx + dy * dy;
+        const int cid = __float_as_int(c.w);
+        bool better = d2 < d[K - 1];
+        if (!better && d2 == d[K - 1]) better = cid < id_at(s, sl[K - 1]);
+        if (better) {
+            nb_insert<K>(s, d, sl, d2, cid, j);
+            n = min(n + 1, K);
+        }
+    }
+    nb = 0;
+    if (n >= K) {
+        thr = fminf(thr, d[K - 1]);
+        const float rr = __fsqrt_rn(thr) * 1.0001f + 2e-3f;
+        prune2 = rr * rr;
+    }
+}
+
+// Squared distance from v to the band [lo, hi) of grid row / column i
+// along one axis; the border bands are open outward (agents outside the
+// grid box are clamped into them).
+__device__ __forceinline__ float band_d2(float v, float v0, float cell, int i, int side) {
+    const float b0 = i == 0 ? -INFINITY : v0 + (float)i * cell;
+    const float b1 = i == side - 1 ? INFINITY : v0 + (float)(i + 1) * cell;
+    const float dd = fmaxf(fmaxf(b0 - v, v - b1), 0.0f);
+    return dd * dd;
+}
+
 // Exact k nearest by ring search over the agent grid.  Rings are walked as
 // one loop over their 8r perimeter cells; a cell is skipped when its box
 // (open toward the outside on the grid border, where clamped agents live)
@@ -474,27 +512,7 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
         const float rr = __fsqrt_rn(thr) * 1.0001f + guard;
         prune2 = rr * rr;
     }
-    auto merge = [&]() {
-        for (int e = 0; e < nb; ++e) {
-            const int j = buf[e * bstride];
-            const float4 c = s.sorted_pr[j];
-            const float dx = c.x - p.x, dy = c.y - p.y;
-            const float d2 = dx * dx + dy * dy;
-            const int cid = __float_as_int(c.w);
-            bool better = d2 < d[K - 1];
-            if (!better && d2 == d[K - 1]) better = cid < id_at(s, sl[K - 1]);
-            if (better) {
-                nb_insert<K>(s, d, sl, d2, cid, j);
-                n = min(n + 1, K);
-            }
-        }
-        nb = 0;
-        if (n >= K) {
-            thr = fminf(thr, d[K - 1]);
-            const float rr = __fsqrt_rn(thr) * 1.0001f + guard;
-            prune2 = rr * rr;
-        }
-    };
+    auto merge = [&]() { knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2); };
     for (int r = 0;; ++r) {
         const int perim = r == 0 ? 1 : 8 * r;
         for (int q = 0; q < perim; ++q) {
@@ -543,10 +561,101 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
     return n;
 }
 
+// Exact k nearest for an agent with a finite seed bound.  Every candidate
+// within the bound lies in the rectangle of cells [grid_cell(p - rr),
+// grid_cell(p + rr)] on both axes (rr = the bound's radius with the 2 mm
+// guard; grid_cell is monotone), so the cells are walked row by row in one
+// pass, without rings.  The walk appends the non-empty, unpruned cells as
+// ranges of sorted positions (consecutive ranges joined) to a per-lane LDS
+// list of R entries (`rbuf`, range r at rbuf[2r * bstride], rbuf[(2r + 1) *
+// bstride]); when it is full, or the walk is over, one flat loop visits all
+// the listed candidates, so the lanes of a wave step through candidates
+// together instead of waiting for each other at every cell.  Candidates
+// within the threshold go to the candidate buffer as in knn_query, merged
+// when it fills and at the end.  The result is again the k smallest (d2, id)
+// among the candidates with eps < d2 <= bound.
+template <int K, int B, int R>   // K == k
+__device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc& E, V2 p, float bound,
+                                                int* buf, int* rbuf, int bstride, int (&sl)[K]) {
+    const GridDesc& g = E.ag;
+    float d[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) { d[i] = INFINITY; sl[i] = -1; }
+    int n = 0, nb = 0;
+    float thr = bound;
+    const float rr = __fsqrt_rn(thr) * 1.0001f + 2e-3f;
+    float prune2 = rr * rr;
+    const int xl = grid_cell(p.x - rr, g.x0, g.inv_cell, g.side);
+    const int xh = grid_cell(p.x + rr, g.x0, g.inv_cell, g.side);
+    const int yl = grid_cell(p.y - rr, g.y0, g.inv_cell, g.side);
+    const int yh = grid_cell(p.y + rr, g.y0, g.inv_cell, g.side);
+    const uint32_t mx0 = part1by1((uint32_t)xl);
+    int x = xl, y = yl;
+    uint32_t mx = mx0, my = part1by1((uint32_t)yl) << 1;
+    float dy2 = band_d2(p.y, g.y0, g.cell, y, g.side);
+    bool walking = true;
+    uint32_t ca = 0, cb = 0;             // open range [ca, cb), empty when ca == cb
+    while (true) {
+        int nr = 0;
+        while (walking && nr < R) {
+            if (band_d2(p.x, g.x0, g.cell, x, g.side) + dy2 <= prune2) {
+                const uint32_t m = E.cell_base + (mx | my);
+                const uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
+                if (a < b) {
+                    if (a == cb) cb = b;
+                    else {
+                        if (ca < cb) {
+                            rbuf[(2 * nr) * bstride] = (int)ca;
+                            rbuf[(2 * nr + 1) * bstride] = (int)cb;
+                            ++nr;
+                        }
+                        ca = a; cb = b;
+                    }
+                }
+            }
+            if (x < xh) { ++x; mx = ((mx | 0xaaaaaaaau) + 1u) & 0x55555555u; }
+            else if (y < yh) {
+                ++y; x = xl; mx = mx0; my = part1by1((uint32_t)y) << 1;
+                dy2 = band_d2(p.y, g.y0, g.cell, y, g.side);
+            } else walking = false;
+        }
+        if (!walking && nr < R && ca < cb) {
+            rbuf[(2 * nr) * bstride] = (int)ca;
+            rbuf[(2 * nr + 1) * bstride] = (int)cb;
+            ++nr;
+            ca = cb;
+        }
+        if (nr > 0) {
+            int r = 0;
+            uint32_t j = (uint32_t)rbuf[0], e = (uint32_t)rbuf[bstride];
+            while (true) {
+                const float2 c = *reinterpret_cast<const float2*>(&s.sorted_pr[j]);
+                const float dx = c.x - p.x, dy = c.y - p.y;
+                const float d2 = dx * dx + dy * dy;
+                if (d2 > kEps && d2 <= thr) {
+                    buf[nb * bstride] = (int)j;
+                    if (++nb == B) knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2);
+                }
+                if (++j == e) {
+                    if (++r == nr) break;
+                    j = (uint32_t)rbuf[(2 * r) * bstride];
+                    e = (uint32_t)rbuf[(2 * r + 1) * bstride];
+                }
+            }
+        }
+        if (!walking && ca == cb) break;
+    }
+    if (nb > 0) knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2);
+    return n;
+}
+
 // --------------------------------------- attraction + steering + neighbours
-// Candidate buffer of the neighbour search: kKnnBuf sorted positions per
-// lane in dynamic LDS (kKnnBuf * blockDim.x ints per block).
+// LDS of the neighbour search, per lane: a candidate buffer of kKnnBuf
+// sorted positions, then kKnnRanges ranges of sorted positions (two ints
+// each) for the seeded walk; (kKnnBuf + 2 kKnnRanges) * blockDim.x ints per
+// block, lane-interleaved.
 static constexpr int kKnnBuf = 16;
+static constexpr int kKnnRanges = 4;
 
 // One sorted agent: attraction point, steering, k nearest.  Returns whether
 // it appended an event (a replan or a destroy), after fencing it device-wide.
@@ -615,7 +724,10 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
     const float bound = knn_seed_bound<K>(s, t, id, pos);
-    const int cnt = knn_query<K, kKnnBuf>(s, E, pos, bound, knn_lds + threadIdx.x, blockDim.x, slot);
+    int* buf = knn_lds + threadIdx.x;
+    const int cnt = bound < INFINITY
+        ? knn_query_seeded<K, kKnnBuf, kKnnRanges>(s, E, pos, bound, buf, buf + kKnnBuf * blockDim.x, blockDim.x, slot)
+        : knn_query<K, kKnnBuf>(s, E, pos, bound, buf, blockDim.x, slot);
 #pragma unroll
     for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
@@ -1395,7 +1507,7 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, hip
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
     if (block != 64 && block != 128) block = 256;
     dim3 g(round8(cdiv(s.n_slots, block))), b(block);
-    const size_t lds = (size_t)kKnnBuf * block * sizeof(int);
+    const size_t lds = (size_t)(kKnnBuf + 2 * kKnnRanges) * block * sizeof(int);
     switch (s.k) {
 #define ECM_KNN_CASE(k) case k: hipLaunchKernelGGL((k_attr_knn<k>), g, b, lds, st, s); break;
         ECM_KNN_CASE(1) ECM_KNN_CASE(2) ECM_KNN_CASE(3) ECM_KNN_CASE(4) ECM_KNN_CASE(5) ECM_KNN_CASE(6)
