@@ -1258,14 +1258,25 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     }
     const int nObst = nc;
     const int cnt = s.nbr_cnt[t];
-    for (int j = 0; j < cnt; ++j) {
-        int sj = s.nbr[(size_t)j * s.n_slots + t];
-        float4 q = s.sorted_pr[sj];
-        float2 qv = sorted_vel(s, sj);
-        V2 cp, cn;
-        agent_constraint(s, position, velocity, rad, q, qv, cp, cn);
-        con_st(lds, nc, tid, cp, cn);
-        ++nc;
+    {
+        // all neighbour loads issued before the first constraint
+        float4 q[K];
+        float2 qv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int sj = j < cnt ? s.nbr[(size_t)j * s.n_slots + t] : t;
+            q[j] = s.sorted_pr[sj];
+            qv[j] = sorted_vel(s, sj);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (j < cnt) {
+                V2 cp, cn;
+                agent_constraint(s, position, velocity, rad, q[j], qv[j], cp, cn);
+                con_st(lds, nc, tid, cp, cn);
+                ++nc;
+            }
+        }
     }
 
     V2 o;
