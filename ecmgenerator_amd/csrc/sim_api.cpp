@@ -61,6 +61,35 @@ extern "C" int ecm_device_available(void) {
     return std::strncmp(pr.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
 }
 
+// Whether a point of the square [sx0, sx1] x [sy0, sy1] can lie within m of
+// the convex hull of the quad q: false only when a supporting line of the
+// hull (through two of its vertices, every vertex on one side) leaves the
+// square grown by m strictly on the other side.  The ray-cast containment
+// test (MathUtility::Contains, UtilityFunctions.cpp:54-86) finds a point
+// farther than m from the hull outside the quad whatever its rounding: the
+// crossings it counts lie on the quad's edges, at least m away in x, and
+// the rounding of their abscissae is far below m.  Points on a vertex's
+// exact y, where the test's open y intervals skip crossings, are covered by
+// the exact-y lists instead.
+static bool hull_may_touch(const double (&q)[4][2], double sx0, double sy0, double sx1, double sy1, double m) {
+    const double X[4] = {sx0 - m, sx1 + m, sx1 + m, sx0 - m};
+    const double Y[4] = {sy0 - m, sy0 - m, sy1 + m, sy1 + m};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const double dx = q[j][0] - q[i][0], dy = q[j][1] - q[i][1];
+            if (i == j || (dx == 0.0 && dy == 0.0)) continue;
+            bool support = true;                   // every vertex left of or on i -> j
+            for (int k = 0; k < 4; ++k)
+                if (dx * (q[k][1] - q[i][1]) - dy * (q[k][0] - q[i][0]) < 0.0) support = false;
+            if (!support) continue;
+            bool apart = true;                     // every grown corner strictly right
+            for (int c = 0; c < 4; ++c)
+                if (dx * (Y[c] - q[i][1]) - dy * (X[c] - q[i][0]) >= 0.0) apart = false;
+            if (apart) return false;
+        }
+    return true;
+}
+
 static GridDesc make_grid(float x0, float y0, float x1, float y1, float cell, int max_side) {
     GridDesc g;
     float ext = std::max(x1 - x0, y1 - y0);
@@ -638,7 +667,8 @@ extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float*
         A[2 * e + 1] = make_float4(v0[0], v0[1], h[2], h[3]);
         B[2 * e + 1] = make_float4(h[4], h[5], v1[0], v1[1]);
     }
-    // bucket grid: mean cell box area decides the bucket edge
+    // bucket grid: a quarter of the root of the mean cell box area, in
+    // [1, 4] m, so that a bucket lists few cells
     double area = 0.0;
     float bx0 = E->x0, by0 = E->y0, bx1 = E->x1, by1 = E->y1;
     std::vector<float> box(4 * (size_t)nc);
@@ -652,7 +682,7 @@ extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float*
         bx0 = std::min(bx0, lx); by0 = std::min(by0, ly); bx1 = std::max(bx1, hx); by1 = std::max(by1, hy);
     }
     float cell = (float)std::sqrt(std::max(area / nc, 1e-4));
-    cell = std::min(std::max(cell, 1.0f), 16.0f);
+    cell = std::min(std::max(cell * 0.25f, 1.0f), 4.0f);
     GridDesc g = make_grid(bx0 - 1.0f, by0 - 1.0f, bx1 + 1.0f, by1 + 1.0f, cell, 4096);
     const int side = g.side;
     std::vector<int> cnt((size_t)side * side + 1, 0);
@@ -662,11 +692,25 @@ extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float*
         cx1 = grid_cell(box[4 * c + 2], g.x0, g.inv_cell, side);
         cy1 = grid_cell(box[4 * c + 3], g.y0, g.inv_cell, side);
     };
+    // A bucket lists a cell only if the cell's convex hull comes within
+    // `hm` of the bucket square (bucket squares as grid_cell assigns points:
+    // [x0 + i cell, x0 + (i + 1) cell), the border ones open outward).
+    const double hm = 0.05 + 1e-5 * std::max(std::max(std::fabs((double)bx0), std::fabs((double)bx1)),
+                                             std::max(std::fabs((double)by0), std::fabs((double)by1)));
+    auto listed = [&](int c, int x, int y) {
+        const double sx0 = x == 0 ? -1e30 : (double)g.x0 + (double)x * g.cell;
+        const double sx1 = x == side - 1 ? 1e30 : (double)g.x0 + (double)(x + 1) * g.cell;
+        const double sy0 = y == 0 ? -1e30 : (double)g.y0 + (double)y * g.cell;
+        const double sy1 = y == side - 1 ? 1e30 : (double)g.y0 + (double)(y + 1) * g.cell;
+        const double q[4][2] = {{A[c].x, A[c].y}, {A[c].z, A[c].w}, {B[c].x, B[c].y}, {B[c].z, B[c].w}};
+        return hull_may_touch(q, sx0, sy0, sx1, sy1, hm);
+    };
     for (int c = 0; c < nc; ++c) {
         int cx0, cx1, cy0, cy1;
         range_of(c, cx0, cx1, cy0, cy1);
         for (int y = cy0; y <= cy1; ++y)
-            for (int x = cx0; x <= cx1; ++x) cnt[(size_t)y * side + x]++;
+            for (int x = cx0; x <= cx1; ++x)
+                if (listed(c, x, y)) cnt[(size_t)y * side + x]++;
     }
     std::vector<int> start((size_t)side * side + 1, 0);
     for (size_t i = 0; i < (size_t)side * side; ++i) start[i + 1] = start[i] + cnt[i];
@@ -676,7 +720,8 @@ extern "C" int ecm_sim_env_set_ecm(ecm_sim* s, int env, int n_vert, const float*
         int cx0, cx1, cy0, cy1;
         range_of(c, cx0, cx1, cy0, cy1);
         for (int y = cy0; y <= cy1; ++y)
-            for (int x = cx0; x <= cx1; ++x) cells[fillp[(size_t)y * side + x]++] = c;
+            for (int x = cx0; x <= cx1; ++x)
+                if (listed(c, x, y)) cells[fillp[(size_t)y * side + x]++] = c;
     }
     // exact-y lists: every distinct vertex y of every cell
     std::vector<std::pair<uint32_t, int>> yc;
