@@ -239,7 +239,7 @@ struct ecm_sim {
     ecm_mlp* log_mlp = nullptr;     // set for the duration of ecm_sim_log_step
 
     // launch shape (results do not depend on it; ecm_sim_autotune picks it)
-    ecm_sim_tuning tune{1.0f, 256, 6, 0.0f};
+    ecm_sim_tuning tune{1.0f, 256, 1, 0.0f};
     // the device part of a step captured as two hipGraphs (up to the replan
     // read-back, and the rest), one pair per position buffer parity,
     // recaptured when anything they were captured with changes
@@ -1529,8 +1529,8 @@ extern "C" int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap)
 static int check_tuning(const ecm_sim_tuning& t) {
     if (!(t.cell_scale >= 0.25f && t.cell_scale <= 4.0f)) return fail(ECM_E_ARG, "cell_scale must be in [0.25, 4]");
     if (t.knn_block != 64 && t.knn_block != 128 && t.knn_block != 256) return fail(ECM_E_ARG, "knn_block must be 64, 128 or 256");
-    if (t.orca_obst_slots != 4 && t.orca_obst_slots != 6 && t.orca_obst_slots != 8)
-        return fail(ECM_E_ARG, "orca_obst_slots must be 4, 6 or 8");
+    if (t.orca_obst_slots != 1 && t.orca_obst_slots != 2 && t.orca_obst_slots != 4)
+        return fail(ECM_E_ARG, "orca_obst_slots must be 1, 2 or 4");
     return 0;
 }
 
@@ -1586,7 +1586,7 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
     if (rc) return rc;
     const float scales[] = {0.6f, 0.8f, 1.0f, 1.25f, 1.6f, 2.0f};
     const int blocks[] = {64, 128, 256};
-    const int slots[] = {4, 6, 8};
+    const int slots[] = {1, 2, 4};
     for (int dim = 0; dim < 3; ++dim) {
         const int nopt = dim == 0 ? 6 : 3;
         for (int o = 0; o < nopt; ++o) {

@@ -775,13 +775,25 @@ __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
 // ------------------------------------------------------------------- ORCA
 struct Con { V2 n, p; };
 
-__device__ __forceinline__ Con con_ld(const float4* lds, int slot, int tid) {
-    float4 c = lds[slot * 64 + tid];
+// Where a lane's constraint slot c lives in its block's LDS tile (float4
+// units): slot-major, c * 64 + tid, for c < main; the lane's slots from
+// `main` on are contiguous at ext + (c - main), its share of the block's
+// spill area.  Tiles without a spill area have main = INT_MAX.
+struct Tile {
+    int tid, main, ext;
+    __device__ __forceinline__ int at(int c) const { return c < main ? c * 64 + tid : ext + (c - main); }
+};
+
+// Spill area of the main ORCA pass: float4 slots per 64-lane block.
+static constexpr int kOrcaSpill = 63;
+
+__device__ __forceinline__ Con con_ld(const float4* lds, const Tile& T, int slot) {
+    float4 c = lds[T.at(slot)];
     Con r; r.n = mk(c.x, c.y); r.p = mk(c.z, c.w);
     return r;
 }
-__device__ __forceinline__ void con_st(float4* lds, int slot, int tid, V2 p, V2 n) {
-    lds[slot * 64 + tid] = make_float4(n.x, n.y, p.x, p.y);
+__device__ __forceinline__ void con_st(float4* lds, const Tile& T, int slot, V2 p, V2 n) {
+    lds[T.at(slot)] = make_float4(n.x, n.y, p.x, p.y);
 }
 // Constraint::Contains, methodB (ORCA.h:49-60)
 __device__ __forceinline__ bool con_contains(const Con& c, V2 q) { return det(right(c.n), c.p - q) <= 0.0f; }
@@ -947,14 +959,14 @@ __device__ void agent_constraint(const DevState& s, V2 position, V2 velocity, fl
 // ORCA::RandomizedLP (ORCA.cpp:428-587) over a virtual list: entries
 // [0, n_direct) at LDS slots [0, n_direct), entries [n_direct, n) at slots
 // [proj_base, ...).  Returns the failing index or n.
-__device__ int lp2d(const float4* lds, int tid, int n, int n_direct, int proj_base, V2 opt, float maxSpeed,
+__device__ int lp2d(const float4* lds, const Tile& T, int n, int n_direct, int proj_base, V2 opt, float maxSpeed,
                     bool useDir, V2& out) {
     if (useDir) out = opt * maxSpeed;
     else if (len(opt) > maxSpeed) out = normalized(opt) * maxSpeed;
     else out = opt;
     if (n == 0) return n;
     for (int i = 0; i < n; ++i) {
-        Con h = con_ld(lds, i < n_direct ? i : proj_base + (i - n_direct), tid);
+        Con h = con_ld(lds, T, i < n_direct ? i : proj_base + (i - n_direct));
         if (con_contains(h, out)) continue;
         V2 dir = right(h.n);
         float dpd = dot(dir, h.p);
@@ -964,7 +976,7 @@ __device__ int lp2d(const float4* lds, int tid, int n, int n_direct, int proj_ba
         float lft = -dpd - sq;
         float rgt = -dpd + sq;
         for (int j = 0; j < i; ++j) {
-            Con hj = con_ld(lds, j < n_direct ? j : proj_base + (j - n_direct), tid);
+            Con hj = con_ld(lds, T, j < n_direct ? j : proj_base + (j - n_direct));
             float den = det(right(h.n), right(hj.n));
             float num = det(right(hj.n), h.p - hj.p);
             if (fabsf(den) <= kEps) {
@@ -991,15 +1003,15 @@ __device__ int lp2d(const float4* lds, int tid, int n, int n_direct, int proj_ba
 
 // ORCA::RandomizedLP3D (ORCA.cpp:592-669); projected agent constraints are
 // written to LDS slots [proj_base, proj_base + i - nObst).
-__device__ void lp3d(float4* lds, int tid, int n, int nObst, int proj_base, float maxSpeed, int failed, V2& out) {
+__device__ void lp3d(float4* lds, const Tile& T, int n, int nObst, int proj_base, float maxSpeed, int failed, V2& out) {
     float maxPen = 0.0f;
     for (int i = failed; i < n; ++i) {
-        const Con ci = con_ld(lds, i, tid);
+        const Con ci = con_ld(lds, T, i);
         V2 dir = right(ci.n);
         if (det(dir, ci.p - out) <= maxPen) continue;
         int np = 0;
         for (int j = nObst; j < i; ++j) {
-            const Con cj = con_ld(lds, j, tid);
+            const Con cj = con_ld(lds, T, j);
             float d = det(right(ci.n), right(cj.n));
             V2 newP;
             if (fabsf(d) <= kEps) {
@@ -1009,11 +1021,11 @@ __device__ void lp3d(float4* lds, int tid, int n, int nObst, int proj_base, floa
                 float t = det(right(cj.n), ci.p - cj.p) / d;
                 newP = ci.p + right(ci.n) * t;
             }
-            con_st(lds, proj_base + np, tid, newP, normalized(cj.n - ci.n));
+            con_st(lds, T, proj_base + np, newP, normalized(cj.n - ci.n));
             ++np;
         }
         const V2 temp = out;
-        if (lp2d(lds, tid, nObst + np, nObst, proj_base, ci.n, maxSpeed, true, out) < nObst + np) out = temp;
+        if (lp2d(lds, T, nObst + np, nObst, proj_base, ci.n, maxSpeed, true, out) < nObst + np) out = temp;
         maxPen = det(dir, ci.p - out);
     }
 }
@@ -1161,13 +1173,13 @@ __device__ __forceinline__ void wave_sum5(uint32_t* ctr, uint32_t v) {
 // RandomizedLP (ORCA.cpp:428-587, useDirOpt false) restricted to the
 // constraints in `mask`, in index order, with the same arithmetic as lp2d.
 // Returns false when the restricted problem is infeasible.
-__device__ bool lp2d_mask(const float4* lds, int tid, uint32_t mask, V2 opt, float maxSpeed, V2& out) {
+__device__ bool lp2d_mask(const float4* lds, const Tile& T, uint32_t mask, V2 opt, float maxSpeed, V2& out) {
     if (len(opt) > maxSpeed) out = normalized(opt) * maxSpeed;
     else out = opt;
     uint32_t done = 0;
     for (uint32_t m = mask; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
-        const Con h = con_ld(lds, i, tid);
+        const Con h = con_ld(lds, T, i);
         if (!con_contains(h, out)) {
             V2 dir = right(h.n);
             float dpd = dot(dir, h.p);
@@ -1177,7 +1189,7 @@ __device__ bool lp2d_mask(const float4* lds, int tid, uint32_t mask, V2 opt, flo
             float lft = -dpd - sq;
             float rgt = -dpd + sq;
             for (uint32_t d = done; d; d &= d - 1) {
-                const Con hj = con_ld(lds, __ffs(d) - 1, tid);
+                const Con hj = con_ld(lds, T, __ffs(d) - 1);
                 float den = det(right(h.n), right(hj.n));
                 float num = det(right(hj.n), h.p - hj.p);
                 if (fabsf(den) <= kEps) {
@@ -1213,8 +1225,14 @@ constexpr float kActiveTol = 1e-3f;   // a constraint is labelled active within 
 // redone by k_orca_ovf with a full tile.  A failed 2D LP parks the agent on
 // the 3D-LP queue; so does a rejected learned proposal (failed index -1:
 // the queue kernel runs the full 2D LP first).
-template <int K, int OSLOTS, bool OVERFLOW, int MODE>
-__device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds, int tid) {
+// SPILL (the main pass): the lane's tile holds OSLOTS + K slots; an agent
+// with more obstacle constraints than that is parked for the overflow pass,
+// and one whose obstacle and agent constraints together exceed it takes the
+// rest from the block's spill area (kOrcaSpill slots, counted by *spill_used)
+// or, when that is used up, is parked too.  Without SPILL the tile holds
+// every constraint (OSLOTS >= the longest obstacle bucket list).
+template <int K, int OSLOTS, bool SPILL, int MODE>
+__device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds, Tile TL, int* spill_used) {
     const float4 pr = s.sorted_pr[t];
     const float4 sv = s.sorted_v[t];
     const float2 v0 = make_float2(sv.x, sv.y);
@@ -1242,14 +1260,14 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         obstacle_query(E, position, range * range, [&](int e, float dd) {
             V2 cp, cn;
             const bool made = obstacle_constraint(s, E, e, position, velocity, rad, cp, cn);
-            if (made && OVERFLOW && nc == OSLOTS) {
+            if (made && SPILL && nc == OSLOTS + K) {
                 s.ovf_agent[wave_alloc(&s.stats[1])] = t;
                 parked = true;
                 return false;
             }
             if (MODE != kPlain) near4_insert(T, dd, e, made ? nc : -1);
             if (made) {
-                con_st(lds, nc, tid, cp, cn);
+                con_st(lds, TL, nc, cp, cn);
                 ++nc;
             }
             return true;
@@ -1258,6 +1276,20 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     }
     const int nObst = nc;
     const int cnt = s.nbr_cnt[t];
+    if (SPILL) {
+        // the learned proposal indexes constraints by the bits of a word
+        const int need = nObst + cnt - (OSLOTS + K);
+        bool park = MODE == kLearned && nObst + cnt > 32;
+        if (!park && need > 0) {
+            const int base = atomicAdd(spill_used, need);
+            if (base + need > kOrcaSpill) park = true;
+            else TL.ext = (OSLOTS + K) * 64 + base;
+        }
+        if (park) {
+            s.ovf_agent[wave_alloc(&s.stats[1])] = t;
+            return;
+        }
+    }
     {
         // all neighbour loads issued before the first constraint
         float4 q[K];
@@ -1273,7 +1305,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
             if (j < cnt) {
                 V2 cp, cn;
                 agent_constraint(s, position, velocity, rad, q[j], qv[j], cp, cn);
-                con_st(lds, nc, tid, cp, cn);
+                con_st(lds, TL, nc, cp, cn);
                 ++nc;
             }
         }
@@ -1291,9 +1323,9 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         for (int i = 0; i < mlp::kObstFeat; ++i)
             if (((pm >> (mlp::kAgentSlots + i)) & 1u) && T.c[i] >= 0) cm |= 1u << T.c[i];
         // verification: optimum of the restricted LP, contained by the rest
-        bool ok = lp2d_mask(lds, tid, cm, mk(pv.x, pv.y), maxSpeed, o);
+        bool ok = lp2d_mask(lds, TL, cm, mk(pv.x, pv.y), maxSpeed, o);
         for (int c = 0; ok && c < nc; ++c)
-            if (!((cm >> c) & 1u) && !con_contains(con_ld(lds, c, tid), o)) ok = false;
+            if (!((cm >> c) & 1u) && !con_contains(con_ld(lds, TL, c), o)) ok = false;
         // per-wavefront counters spread over 64 slots (one address each
         // would serialize 15k atomics per launch)
         uint32_t* ws = s.wstats + 4 * (blockIdx.x & 63);
@@ -1305,14 +1337,14 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
             s.q3d_agent[q] = t;
             s.q3d_info[q] = make_int4(nc, nObst, -1, 0);
             float4* dst = s.q3d_con + (size_t)q * s.con_stride;
-            for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
+            for (int c = 0; c < nc; ++c) dst[c] = lds[TL.at(c)];
             return;
         }
         integrate(s, t, position, velocity, o);
         return;
     }
 
-    int fail = lp2d(lds, tid, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
+    int fail = lp2d(lds, TL, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
     if (fail < nc) {
         // RandomizedLP3D runs in k_lp3d on a compacted queue: park the
         // constraints, the 2D result and the failing index
@@ -1321,7 +1353,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         s.q3d_info[q] = make_int4(nc, nObst, fail, 0);
         s.q3d_vel[q] = make_float2(o.x, o.y);
         float4* dst = s.q3d_con + (size_t)q * s.con_stride;
-        for (int c = 0; c < nc; ++c) dst[c] = lds[c * 64 + tid];
+        for (int c = 0; c < nc; ++c) dst[c] = lds[TL.at(c)];
         return;
     }
     if (MODE == kLog) {
@@ -1332,7 +1364,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
             uint32_t act = 0, valid = 0;
             bool repr = true;
             for (int c = 0; c < nc; ++c) {
-                const Con h = con_ld(lds, c, tid);
+                const Con h = con_ld(lds, TL, c);
                 const bool a = det(right(h.n), h.p - o) >= -kActiveTol;
                 int slot = -1;
                 if (c >= nObst) {
@@ -1356,17 +1388,22 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
     integrate(s, t, position, velocity, o);
 }
 
-// Main ORCA pass: one agent per lane in Morton order, a small constraint
-// tile (OSLOTS obstacle slots: 4, 6 or 8, picked by the autotuner) so that
-// more waves fit per CU.
+// Main ORCA pass: one agent per lane in Morton order.  The constraint tile
+// is small so that more waves fit per CU: OSLOTS + K slots per lane (OSLOTS:
+// 1, 2 or 4, picked by the autotuner) and a spill area of kOrcaSpill slots
+// for the block's few lanes with more constraints; its last float4 holds the
+// spill counter.  12 KiB at OSLOTS 1, K 10.
 template <int K, int MODE, int OSLOTS>
 __global__ __launch_bounds__(64) void k_orca(DevState s) {
-    __shared__ float4 lds[(OSLOTS + K) * 64];
-    const uint32_t n_active = s.cell_start[s.n_keys];
+    __shared__ float4 lds[(OSLOTS + K) * 64 + kOrcaSpill + 1];
+    int* spill_used = reinterpret_cast<int*>(&lds[(OSLOTS + K) * 64 + kOrcaSpill]);
     const int tid = threadIdx.x;
+    if (tid == 0) *spill_used = 0;
+    __syncthreads();
+    const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * 64 + tid;
     if (t >= (int)n_active) return;
-    orca_agent<K, OSLOTS, true, MODE>(s, t, lds, tid);
+    orca_agent<K, OSLOTS, true, MODE>(s, t, lds, Tile{tid, OSLOTS + K, 0}, spill_used);
 }
 
 // Agents with more obstacle constraints than the main tile holds, with a tile for
@@ -1381,7 +1418,7 @@ __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     // block i % gridDim), so a short queue spreads over many CUs instead of
     // filling the lanes of a few divergent wavefronts
     for (uint32_t i = tid * gridDim.x + blockIdx.x; i < nq; i += gridDim.x * 64)
-        orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, tid);
+        orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, Tile{tid, INT_MAX, 0}, nullptr);
 }
 
 // RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed
@@ -1392,12 +1429,13 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     __shared__ float4 lds[(MAXO + 2 * K) * 64];
     const uint32_t nq = s.stats[0];
     const int tid = threadIdx.x;
+    const Tile T{tid, INT_MAX, 0};
     // entries go lane-major across the blocks, as in k_orca_ovf
     for (uint32_t q = tid * gridDim.x + blockIdx.x; q < nq; q += gridDim.x * 64) {
         const int t = s.q3d_agent[q];
         int4 info = s.q3d_info[q];
         const float4* src = s.q3d_con + (size_t)q * s.con_stride;
-        for (int c = 0; c < info.x; ++c) lds[c * 64 + tid] = src[c];
+        for (int c = 0; c < info.x; ++c) lds[T.at(c)] = src[c];
         const float4 pr = s.sorted_pr[t];
         const float4 sv = s.sorted_v[t];
         const float2 v0 = make_float2(sv.x, sv.y);
@@ -1405,12 +1443,12 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
         V2 o;
         if (info.z < 0) {
             const float2 pv = s.pref_vel_s[t];
-            info.z = lp2d(lds, tid, info.x, info.x, 0, mk(pv.x, pv.y), maxSpeed, false, o);
+            info.z = lp2d(lds, T, info.x, info.x, 0, mk(pv.x, pv.y), maxSpeed, false, o);
         } else {
             const float2 ov = s.q3d_vel[q];
             o = mk(ov.x, ov.y);
         }
-        if (info.z < info.x) lp3d(lds, tid, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
+        if (info.z < info.x) lp3d(lds, T, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
         integrate(s, t, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
     }
 }
@@ -1500,9 +1538,9 @@ static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStr
 template <int K>
 static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, hipStream_t st) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
-    if (oslots <= 4) launch_orca_main<K, 4>(s, mode, g, b, st);
-    else if (oslots <= 6) launch_orca_main<K, 6>(s, mode, g, b, st);
-    else launch_orca_main<K, 8>(s, mode, g, b, st);
+    if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
+    else if (oslots <= 2) launch_orca_main<K, 2>(s, mode, g, b, st);
+    else launch_orca_main<K, 4>(s, mode, g, b, st);
     if (maxo <= 16) {
         hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 16>), gq, b, 0, st, s);

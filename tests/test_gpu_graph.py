@@ -76,7 +76,7 @@ def test_autotune_changes_speed_not_results():
     b = make(sc, agents, True)
     t = b.autotune(steps=2)
     print(f"autotune picked {t}", flush=True)
-    assert t["knn_block"] in (64, 128, 256) and t["orca_obst_slots"] in (4, 6, 8) and t["best_ms"] > 0
+    assert t["knn_block"] in (64, 128, 256) and t["orca_obst_slots"] in (1, 2, 4) and t["best_ms"] > 0
     n = b.stats()["steps"]
     a.Update(steps=n)
     same(a, b)
