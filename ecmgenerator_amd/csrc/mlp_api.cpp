@@ -275,7 +275,7 @@ extern "C" int ecm_mlp_forward(ecm_mlp* m, const float* x, int n, float* y) {
     int rc = ensure_ws(m, n);
     if (!rc) rc = stage_host_batch(m, x, nullptr, nullptr, n);
     if (rc) return rc;
-    HIPCHK(launch_mlp_infer(m->xb.p, nullptr, n, m->frags.p, m->pbias.p, 0.0f, m->lb.p, nullptr, m->y.p, m->n_cu,
+    HIPCHK(launch_mlp_infer(m->xb.p, 0, nullptr, n, m->frags.p, m->pbias.p, 0.0f, m->lb.p, nullptr, m->y.p, m->n_cu,
                             m->stream));
     HIPCHK(hipMemcpyAsync(y, m->y.p, (size_t)n * kOut * sizeof(float), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));

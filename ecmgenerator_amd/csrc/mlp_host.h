@@ -13,8 +13,11 @@
 namespace ecm {
 namespace mlp {
 
-hipError_t launch_mlp_infer(const void* feat, const uint32_t* n_dev, int n_host, const void* frags, const float* bias,
-                            float theta, uint32_t* mask, float2* vel, float* raw, int n_cu, hipStream_t st);
+// feat: [n][8] chunks of 8 BF16 per agent, or with `tiled` chunk c of agent
+// i at ((i / 32) * 8 + c) * 32 + i % 32.
+hipError_t launch_mlp_infer(const void* feat, int tiled, const uint32_t* n_dev, int n_host, const void* frags,
+                            const float* bias, float theta, uint32_t* mask, float2* vel, float* raw, int n_cu,
+                            hipStream_t st);
 hipError_t launch_gemm(int M, int N, int K, const __bf16* A, long sam, long sak, const __bf16* B, long sbn, long sbk,
                        float* C, long scm, long scn, int accumulate, int splitk, hipStream_t st);
 hipError_t launch_bias_silu(float* Z, const float* bias, __bf16* H, int B, int N, hipStream_t st);

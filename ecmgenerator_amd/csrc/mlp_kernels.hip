@@ -63,7 +63,8 @@ __device__ __forceinline__ void hidden_layer(const bf16x8* sw, int frag0, const 
 }
 
 struct InferArgs {
-    const bf16x8* feat;       // [n][8]: 64 bf16 features per agent
+    const bf16x8* feat;       // 64 bf16 features per agent: [n][8], or tiled (below)
+    int tiled;                // chunk c of agent i at ((i / 32) * 8 + c) * 32 + i % 32
     const uint32_t* n_dev;    // agent count on the device, or null
     int n_host;               // agent count when n_dev is null
     const bf16x8* frags;      // [kNFrags][64] packed A fragments
@@ -100,7 +101,10 @@ __global__ __launch_bounds__(1024) void k_mlp_infer(InferArgs a) {
         // layer 1: 64 inputs straight from HBM as B operands
         bf16x8 x[4], h[8], h2[8];
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) x[kc] = ok ? a.feat[(size_t)agent * 8 + 2 * kc + g] : zero;
+        for (int kc = 0; kc < 4; ++kc) {
+            const size_t at = a.tiled ? ((size_t)tile * 8 + 2 * kc + g) * 32 + col : (size_t)agent * 8 + 2 * kc + g;
+            x[kc] = ok ? a.feat[at] : zero;
+        }
         hidden_layer<4>(sw, kFragL1, sb + kBiasL1, g, lane, x, h);
         hidden_layer<8>(sw, kFragL2, sb + kBiasL2, g, lane, h, h2);
         hidden_layer<8>(sw, kFragL3, sb + kBiasL3, g, lane, h2, h);
@@ -332,9 +336,10 @@ __global__ void k_pack(const float* P, bf16x8* frags, float* bias) {
 // ------------------------------------------------------------- launchers
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-hipError_t launch_mlp_infer(const void* feat, const uint32_t* n_dev, int n_host, const void* frags, const float* bias,
-                            float theta, uint32_t* mask, float2* vel, float* raw, int n_cu, hipStream_t st) {
-    InferArgs a{(const bf16x8*)feat, n_dev, n_host, (const bf16x8*)frags, bias, theta, mask, vel, raw};
+hipError_t launch_mlp_infer(const void* feat, int tiled, const uint32_t* n_dev, int n_host, const void* frags,
+                            const float* bias, float theta, uint32_t* mask, float2* vel, float* raw, int n_cu,
+                            hipStream_t st) {
+    InferArgs a{(const bf16x8*)feat, tiled, n_dev, n_host, (const bf16x8*)frags, bias, theta, mask, vel, raw};
     hipLaunchKernelGGL(k_mlp_infer, dim3(n_cu), dim3(1024), 0, st, a);
     return hipGetLastError();
 }

@@ -1330,7 +1330,7 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
     if (mode == 2) {
         // learned proposal: features, network, then the verifying ORCA pass
         HIPCHK(launch_features(d, s->stream));
-        HIPCHK(mlp::launch_mlp_infer(s->feat.p, s->cell_start.p + s->n_keys, 0, s->mlp->frags.p, s->mlp->pbias.p,
+        HIPCHK(mlp::launch_mlp_infer(s->feat.p, 1, s->cell_start.p + s->n_keys, 0, s->mlp->frags.p, s->mlp->pbias.p,
                                      s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu, s->stream));
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
@@ -1783,8 +1783,9 @@ extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out, int n) {
 // ------------------------------------------------------ learned proposal
 static int ensure_learned_buffers(ecm_sim* s) {
     const size_t N = (size_t)s->prm.max_agents;
-    if (s->feat.n < N * 8) {
-        int rc = s->feat.alloc(N * 8);
+    const size_t NF = (N + 31) / 32 * 32 * 8;      // tiles of 32 agents
+    if (s->feat.n < NF || s->mlp_mask.n < N) {
+        int rc = s->feat.alloc(NF);
         if (!rc) rc = s->mlp_mask.alloc(N);
         if (rc) return rc;
     }

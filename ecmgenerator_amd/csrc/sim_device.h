@@ -140,7 +140,8 @@ struct DevState {
     int con_stride;         // MAXO + k
 
     // ---- learned proposal ----
-    uint4* feat;                // [n_slots][8] BF16 network inputs by sorted position
+    uint4* feat;                // BF16 network inputs by sorted position t: chunk c (8 BF16) at
+                                // ((t / 32) * 8 + c) * 32 + t % 32; n_slots rounded up to 32
     const uint32_t* mlp_mask;   // [n_slots] predicted active slots by sorted position
     // training-sample log (one plain step that records its LP solutions)
     uint4* log_x;               // [log_cap][8] BF16 features

@@ -1079,10 +1079,11 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
 }
 
-// The 64 BF16 features of one agent (layout in mlp.h) to out[0..7].
+// The 64 BF16 features of one agent (layout in mlp.h): its 8 chunks of 8 BF16 to out[0], out[stride],
+// ... out[7 * stride].
 template <int K>
 __device__ void agent_features(const DevState& s, const EnvDesc& E, int t, V2 position, V2 velocity, float rad,
-                               float2 pv, int cnt, const Near4& T, uint4* out) {
+                               float2 pv, int cnt, const Near4& T, uint4* out, int stride) {
     float f[mlp::kInPad];
 #pragma unroll
     for (int i = 0; i < mlp::kInPad; ++i) f[i] = 0.0f;
@@ -1115,8 +1116,8 @@ __device__ void agent_features(const DevState& s, const EnvDesc& E, int t, V2 po
     f[60] = feat_clamp(rad * mlp::kRadScale);
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        out[c] = make_uint4(pack_bf16x2(f[8 * c + 0], f[8 * c + 1]), pack_bf16x2(f[8 * c + 2], f[8 * c + 3]),
-                            pack_bf16x2(f[8 * c + 4], f[8 * c + 5]), pack_bf16x2(f[8 * c + 6], f[8 * c + 7]));
+        out[c * stride] = make_uint4(pack_bf16x2(f[8 * c + 0], f[8 * c + 1]), pack_bf16x2(f[8 * c + 2], f[8 * c + 3]),
+                                     pack_bf16x2(f[8 * c + 4], f[8 * c + 5]), pack_bf16x2(f[8 * c + 6], f[8 * c + 7]));
 }
 
 // Obstacle query with the reference's filter (Simulator.cpp:266-287) over
@@ -1360,7 +1361,7 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         // training sample: features, active slots at the LP velocity, v
         const uint32_t idx = wave_alloc(s.log_n);
         if (idx < s.log_cap) {
-            agent_features<K>(s, E, t, position, velocity, rad, pv, cnt, T, s.log_x + (size_t)idx * 8);
+            agent_features<K>(s, E, t, position, velocity, rad, pv, cnt, T, s.log_x + (size_t)idx * 8, 1);
             uint32_t act = 0, valid = 0;
             bool repr = true;
             for (int c = 0; c < nc; ++c) {
@@ -1463,10 +1464,13 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
     const float4 pr = s.sorted_pr[t];
     const float4 sv = s.sorted_v[t];
     const float2 v0 = make_float2(sv.x, sv.y);
-    uint4* out = s.feat + (size_t)t * 8;
+    // tiles of 32 agents, chunk-major: chunk c of agent t at
+    // ((t / 32) * 8 + c) * 32 + t % 32, so each store (and the network's
+    // loads) covers 512 contiguous bytes per 32 lanes
+    uint4* out = s.feat + ((size_t)(t >> 5) * 8) * 32 + (t & 31);
     if (s.dead_s[t]) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) out[c] = make_uint4(0, 0, 0, 0);
+        for (int c = 0; c < 8; ++c) out[c * 32] = make_uint4(0, 0, 0, 0);
         return;
     }
     const V2 position = mk(pr.x, pr.y);
@@ -1483,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_features(DevState s) {
             return true;
         });
     }
-    agent_features<K>(s, E, t, position, velocity, rad, s.pref_vel_s[t], s.nbr_cnt[t], T, out);
+    agent_features<K>(s, E, t, position, velocity, rad, s.pref_vel_s[t], s.nbr_cnt[t], T, out, 32);
 }
 
 // neighbour lists per slot id (debug / parity): [id*k + j] = neighbour id
