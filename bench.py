@@ -141,9 +141,11 @@ def main():
         "orca_lp_integrate": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
     }
     achieved = bytes_per_agent[names[dom]] * n_per_step / (kt_mean[dom] * 1e-3) / 1e9
-    traffic, tsrc = _profiled_traffic(name, names[dom])
+    traffic, tsrc = _profiled(name, names[dom], "traffic")
+    valu, _ = _profiled(name, names[dom], "valu_busy")
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None if traffic is None else int(traffic),
+            "traffic_source": tsrc, "valu_busy": valu,
             "kernel": names[dom], "algorithmic_bytes_per_launch": int(bytes_per_agent[names[dom]] * n_per_step),
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
@@ -187,19 +189,21 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def _profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE / WRITE_SIZE, calibrated against a
-    copy of known size), or (None, None) when there is none."""
+def _profiled(workload, kernel, what):
+    """Value `what` ("traffic": HBM bytes per launch from FETCH_SIZE /
+    WRITE_SIZE calibrated against a copy of known size; "valu_busy": the
+    fraction of the chip's VALU issue slots the kernel kept busy) of `kernel`
+    from the committed rocprofv3 --pmc summary of this workload, or
+    (None, None) when there is none."""
     import re
     path = os.path.join(ROOT, "profiles", f"pmc_{workload}.md")
     if not os.path.exists(path):
         return None, None
-    m = re.search(r"<!-- traffic: (\{.*?\}) -->", open(path).read())
+    m = re.search(r"<!-- " + what + r": (\{.*?\}) -->", open(path).read())
     if not m:
         return None, None
     t = json.loads(m.group(1)).get(kernel)
-    return (int(t), os.path.relpath(path, ROOT)) if t else (None, None)
+    return (t, os.path.relpath(path, ROOT)) if t else (None, None)
 
 
 def _first_step_after_load(sim, snap):
