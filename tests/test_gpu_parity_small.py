@@ -52,10 +52,12 @@ def test_parity_dense_crowd():
 
 
 def test_parity_pillar_forest_obstacle_overflow():
-    """1 m pillars on a 3.5 m lattice: about 40% of the agents get more
-    obstacle constraints (up to 14) than the main ORCA tile holds, so the
-    overflow queue kernel runs; the result must still match the oracle bit
-    for bit."""
+    """1 m pillars on a 3.5 m lattice: about 40% of the agents get several
+    obstacle constraints (up to 14).  Those whose constraints outnumber
+    their slots in the main ORCA tile take slots from their workgroup's
+    spill area, and those with more obstacle constraints than the tile, or
+    whose workgroup's spill area is used up, go to the overflow queue
+    kernel; the result must still match the oracle bit for bit."""
     from oracle import oracle_py as O
     from ecmgenerator_amd import scenarios as S
     half = 20.0
