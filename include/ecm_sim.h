@@ -109,13 +109,14 @@ int ecm_sim_set_graph(ecm_sim* s, int on);
 int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
 
 /* Launch shape; no choice changes any result.  cell_scale multiplies the
- * agent grid cell edge (about one agent per cell at 1), knn_block is the
- * workgroup size of the attraction + k-nearest kernel (64, 128, 256), and
- * orca_obst_slots the constraint slots per agent of the main ORCA tile
- * beyond k (1, 2, 4): agents with more constraints borrow from their
- * workgroup's small spill area, and those with more obstacle constraints
- * than the tile holds, or whose workgroup's spill area is used up, take the
- * overflow pass.  best_ms: step time the autotuner measured (0 before). */
+ * agent grid cell edge (about one agent per cell at 1; default 1.25),
+ * knn_block is the workgroup size of the attraction + k-nearest kernel (64,
+ * 128, 256; default 256), and orca_obst_slots the constraint slots per
+ * agent of the main ORCA tile beyond k (1, 2, 4; default 1): agents with
+ * more constraints borrow from their workgroup's small spill area, and
+ * those with more obstacle constraints than the tile holds, or whose
+ * workgroup's spill area is used up, take the overflow pass.  best_ms: step
+ * time the autotuner measured (0 before). */
 typedef struct ecm_sim_tuning {
     float cell_scale;
     int knn_block;
