@@ -411,13 +411,14 @@ __device__ __forceinline__ void nb_insert(const DevState& s, float (&d)[K], int 
 // positions through fwd, when all of them are still on the grid of its
 // environment and none is the agent itself (any k distinct candidates bound
 // the k-th distance from above; fwd is one-to-one).  INFINITY when there is
-// no such seed.
+// no such seed.  The seeds (this step's sorted positions) and their squared
+// distances are left in ss / sd, unordered.
 template <int K>   // K == k
-__device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id, V2 p) {
+__device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id, V2 p, float (&sd)[K], int (&ss)[K]) {
     const int q = s.prev_of[t];
     if (q < 0 || q >= s.n_slots || s.nbr_cnt_prev[q] < K) return INFINITY;
     const int env = s.agent_env ? __float_as_int(s.sorted_v[t].w) : 0;
-    int ns[K];
+    int* ns = ss;
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -438,6 +439,7 @@ __device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id
         ok &= d2 > kEps;
         if (s.agent_env) ok &= __float_as_int(s.sorted_v[ns[j]].w) == env;
         bound = fmaxf(bound, d2);
+        sd[j] = d2;
     }
     return ok ? bound : INFINITY;
 }
@@ -561,6 +563,24 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
     return n;
 }
 
+// Orders (d, sl) by (d2, id) in place: odd-even transposition sort, K
+// rounds of compare-exchanges; ids are loaded only when two distances tie.
+template <int K>
+__device__ __forceinline__ void knn_sort(const DevState& s, float (&d)[K], int (&sl)[K]) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+#pragma unroll
+        for (int i = r & 1; i + 1 < K; i += 2) {
+            bool sw = d[i + 1] < d[i];
+            if (d[i + 1] == d[i]) sw = id_at(s, sl[i + 1]) < id_at(s, sl[i]);
+            const float da = d[i], db = d[i + 1];
+            const int sa = sl[i], sb = sl[i + 1];
+            d[i] = sw ? db : da; d[i + 1] = sw ? da : db;
+            sl[i] = sw ? sb : sa; sl[i + 1] = sw ? sa : sb;
+        }
+    }
+}
+
 // Exact k nearest for an agent with a finite seed bound.  Every candidate
 // within the bound lies in the rectangle of cells [grid_cell(p - rr),
 // grid_cell(p + rr)] on both axes (rr = the bound's radius with the 2 mm
@@ -572,16 +592,17 @@ __device__ __forceinline__ int knn_query(const DevState& s, const EnvDesc& E, V2
 // the listed candidates, so the lanes of a wave step through candidates
 // together instead of waiting for each other at every cell.  Candidates
 // within the threshold go to the candidate buffer as in knn_query, merged
-// when it fills and at the end.  The result is again the k smallest (d2, id)
-// among the candidates with eps < d2 <= bound.
+// when it fills and at the end.  The list starts as the k seeds (d, sl from
+// knn_seed_bound), sorted: they are candidates within the bound, so a
+// candidate the walk meets that is in the list is skipped, and only the few
+// new ones are inserted.  The result is again the k smallest (d2, id) among
+// the candidates with eps < d2 <= bound.
 template <int K, int B, int R>   // K == k
 __device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc& E, V2 p, float bound,
-                                                int* buf, int* rbuf, int bstride, int (&sl)[K]) {
+                                                int* buf, int* rbuf, int bstride, float (&d)[K], int (&sl)[K]) {
     const GridDesc& g = E.ag;
-    float d[K];
-#pragma unroll
-    for (int i = 0; i < K; ++i) { d[i] = INFINITY; sl[i] = -1; }
-    int n = 0, nb = 0;
+    knn_sort<K>(s, d, sl);
+    int n = K, nb = 0;
     float thr = bound;
     const float rr = __fsqrt_rn(thr) * 1.0001f + 2e-3f;
     float prune2 = rr * rr;
@@ -633,8 +654,13 @@ __device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc
                 const float dx = c.x - p.x, dy = c.y - p.y;
                 const float d2 = dx * dx + dy * dy;
                 if (d2 > kEps && d2 <= thr) {
-                    buf[nb * bstride] = (int)j;
-                    if (++nb == B) knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2);
+                    bool listed = false;
+#pragma unroll
+                    for (int i = 0; i < K; ++i) listed |= sl[i] == (int)j;
+                    if (!listed) {
+                        buf[nb * bstride] = (int)j;
+                        if (++nb == B) knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2);
+                    }
                 }
                 if (++j == e) {
                     if (++r == nr) break;
@@ -723,10 +749,12 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
-    const float bound = knn_seed_bound<K>(s, t, id, pos);
+    float sd[K];
+    const float bound = knn_seed_bound<K>(s, t, id, pos, sd, slot);
     int* buf = knn_lds + threadIdx.x;
     const int cnt = bound < INFINITY
-        ? knn_query_seeded<K, kKnnBuf, kKnnRanges>(s, E, pos, bound, buf, buf + kKnnBuf * blockDim.x, blockDim.x, slot)
+        ? knn_query_seeded<K, kKnnBuf, kKnnRanges>(s, E, pos, bound, buf, buf + kKnnBuf * blockDim.x, blockDim.x,
+                                                   sd, slot)
         : knn_query<K, kKnnBuf>(s, E, pos, bound, buf, blockDim.x, slot);
 #pragma unroll
     for (int i = 0; i < K; ++i)
