@@ -171,7 +171,7 @@ struct ecm_sim {
     bool sorted_valid = false;                  // the last step's sorted outputs hold the state
     bool id_current = true;                     // the slot-indexed arrays hold it
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
-    DBuf<int> fwd, prev_of;            // previous <-> this step's sorted positions
+    DBuf<int> fwd, prev_of, pos_of;    // previous <-> this step's sorted positions; id -> last one
 
     // Per-step counters and the event list share one device block, so one
     // copy brings both back: words [0..7] are the counters ([0] events,
@@ -299,7 +299,7 @@ static void free_all(ecm_sim* s) {
     s->path_off.release(); s->path_cnt.release(); s->path_pool.release();
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); }
-    s->fwd.release(); s->prev_of.release();
+    s->fwd.release(); s->prev_of.release(); s->pos_of.release();
     s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     for (auto& e : s->envs) e->release();
@@ -412,6 +412,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     }
     if (!rc) rc |= s->fwd.alloc(N);
     if (!rc) rc |= s->prev_of.alloc(N);
+    if (!rc) rc |= s->pos_of.alloc(N);
     if (!rc) rc |= s->xfer.alloc(8 + 4 * N);
     if (!rc) rc |= s->wstats.alloc(256);
     if (!rc) rc |= s->acc.alloc(16);
@@ -439,6 +440,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         }
         rc |= fill_dev_bytes(s->fwd.p, 0xff, N * sizeof(int), s->stream);
         rc |= fill_dev_bytes(s->prev_of.p, 0xff, N * sizeof(int), s->stream);
+        rc |= fill_dev_bytes(s->pos_of.p, 0xff, N * sizeof(int), s->stream);
         rc |= zero_dev(s->path_off.p, N * sizeof(int), s->stream);
         rc |= zero_dev(s->radius.p, N * sizeof(float), s->stream);
         rc |= zero_dev(s->pref_speed.p, N * sizeof(float), s->stream);
@@ -1136,7 +1138,7 @@ static DevState make_state(ecm_sim* s) {
     d.sorted_pr = s->sorted_pr[s->cur].p; d.sorted_v = s->sorted_v[s->cur].p;
     d.nbr = s->nbr[s->cur].p; d.nbr_cnt = s->nbr_cnt[s->cur].p;
     d.nbr_prev = s->nbr[s->cur ^ 1].p; d.nbr_cnt_prev = s->nbr_cnt[s->cur ^ 1].p;
-    d.fwd = s->fwd.p; d.prev_of = s->prev_of.p;
+    d.fwd = s->fwd.p; d.prev_of = s->prev_of.p; d.pos_of = s->pos_of.p;
     d.events = reinterpret_cast<int4*>(s->xfer.p + 8); d.events_n = s->xfer.p;
     d.stats = s->xfer.p + 2;
     d.wstats = s->wstats.p;

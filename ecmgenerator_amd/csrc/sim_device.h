@@ -114,6 +114,7 @@ struct DevState {
     const int* nbr_cnt_prev;
     int* fwd;                 // [n_slots] previous sorted position -> this step's, -1: left the grid
     int* prev_of;             // [n_slots] this step's sorted position -> previous, -1: none
+    int* pos_of;              // [n_slots] agent id -> its sorted position in the last step (k_unsort)
 
     // ---- events ----
     // One list per step, read back by the host after the neighbour pass:

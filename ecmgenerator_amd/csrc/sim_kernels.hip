@@ -86,9 +86,14 @@ __device__ __forceinline__ void step_begin(uint32_t* counters, uint32_t* wstats,
 // Histogram pass, entered from the slot-indexed arrays: one element per
 // slot id; the atomic's return value is the agent's arrival order in its cell.
 __global__ __launch_bounds__(256) void k_grid_count(DevState s) {
-    if (blockIdx.x == 0) step_begin(s.events_n, s.wstats, s.acc);
+    const uint32_t n_old = (uint32_t)s.acc[13];
+    if (blockIdx.x == 0) {
+        step_begin(s.events_n, s.wstats, s.acc);
+        if (threadIdx.x == 0) s.acc[14] = n_old;
+    }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
+    if (i < (int)n_old) s.fwd[i] = -1;        // set again by the scatter for agents found there
     if (!s.active[i]) { s.key[i] = kInvalidKey; return; }
     const uint32_t m = cell_key(s.envs[slot_env(s, i)], s.pos_in[i]);
     s.key[i] = m;
@@ -184,8 +189,12 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* start, const uint32_
 }
 
 // Counting-sort scatter from the slot-indexed arrays: the sorted position
-// is the cell's start plus the arrival order the count pass recorded.  No
-// neighbour seeds after the host changed the agents (prev_of = -1).
+// is the cell's start plus the arrival order the count pass recorded.  An
+// agent that was at sorted position u of the last step (pos_of, written by
+// k_unsort before the host changed anything; checked against the id stored
+// there) keeps its neighbour seeds: prev_of and fwd link u and its new
+// position.  Any k distinct agents bound the k-th distance, so seeds stay
+// safe after spawns, removals, writes and loads.
 __global__ void k_grid_scatter(DevState s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n_slots) return;
@@ -196,7 +205,10 @@ __global__ void k_grid_scatter(DevState s) {
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
     s.sorted_v[slot] = make_float4(v.x, v.y, s.pref_speed[i], __int_as_float(slot_env(s, i)));
     s.attr_s[slot] = s.attraction[i];
-    s.prev_of[slot] = -1;
+    const int u = s.pos_of[i];
+    const bool was = u >= 0 && u < (int)(uint32_t)s.acc[14] && __float_as_int(s.sorted_pr_old[u].w) == i;
+    s.prev_of[slot] = was ? u : -1;
+    if (was) s.fwd[u] = (int)slot;
 }
 
 // Counting-sort scatter from the last step's sorted outputs: reads and
@@ -226,6 +238,7 @@ __global__ void k_unsort(DevState s) {
     if (t >= (int)(uint32_t)s.acc[13]) return;         // the last step's active count
     if (s.dead_s_old[t]) return;                       // written when it was destroyed
     const int id = __float_as_int(s.sorted_pr_old[t].w);
+    s.pos_of[id] = t;
     s.pos_in[id] = s.pos_s_old[t];
     s.vel_in[id] = s.vel_s_old[t];
     s.attraction[id] = s.attr_s_old[t];
@@ -731,7 +744,7 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
         s.vel_in[id] = make_float2(sv.x, sv.y);
         s.attraction[id] = make_float2(attr.x, attr.y);
         const int qp = s.prev_of[t];
-        if (qp >= 0) {
+        if (s.sorted_in && qp >= 0) {
             s.pref_vel[id] = s.pref_vel_s_old[qp];
             if (s.v_orca) s.v_orca[id] = s.v_orca_s_old[qp];
         }
