@@ -67,12 +67,14 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
-    # C++ driver of the Simulator facade (run by a GPU test)
-    drv = os.path.join(HERE, "..", "tests", "cpp", "facade_drive.cpp")
-    if os.path.exists(drv):
+    # C++ drivers of the Simulator facade (run by GPU tests)
+    for name in ("facade_drive", "areas_drive"):
+        drv = os.path.join(HERE, "..", "tests", "cpp", name + ".cpp")
+        if not os.path.exists(drv):
+            continue
         outdir = os.path.join(HERE, "..", "tests", "cpp", "_build")
         os.makedirs(outdir, exist_ok=True)
-        out = os.path.join(outdir, "facade_drive")
+        out = os.path.join(outdir, name)
         cmd = [HIPCC, "-O2", "-std=c++17", drv, "-o", out, "-L", HERE, "-lecmcrowd",
                "-Wl,-rpath,$ORIGIN/../../../ecmgenerator_amd"]
         r = subprocess.run(cmd, capture_output=True, text=True)

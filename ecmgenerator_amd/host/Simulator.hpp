@@ -9,9 +9,22 @@
 // Differences: the ECM, planner and environment arrive as an ecm_scene
 // (include/ecm_scene.h) instead of the reference's Boost-backed classes, and
 // component writes go through the explicit Set* calls.
+//
+// Spawn and goal areas (Simulator.h:92-104, Area.h) follow the reference's
+// UpdateSpawnAreas (Simulator.cpp:494-536), with these differences: random
+// positions come from a seeded std::mt19937 (SetAreaSeed) instead of rand();
+// a connection to a goal area that does not exist, or with a rate <= 0,
+// spawns nothing (the reference default-constructs the goal area with
+// uninitialised extents, and divides 0 by a zero rate); Deconnect removes
+// the connection's rate and timer with it (the reference keeps them, which
+// shifts the later connections' rates); obstacle areas (ECM edits) are not
+// provided.
 #pragma once
 
+#include <cstdint>
 #include <cstring>
+#include <map>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -29,6 +42,32 @@ struct PositionComponent { float x; float y; };
 struct VelocityComponent { float dx; float dy; };
 struct ClearanceComponent { float clearance; };
 struct SpeedComponent { float speed; };
+
+// Area.h:11-103
+enum SimAreaType { NONE, WALKABLE, SPAWN, GOAL, OBSTACLE };
+struct SpawnConfiguration {
+    float preferredSpeedMin = 5.0f;
+    float preferredSpeedMax = 10.0f;
+    float clearanceMin = 5.0f;
+    float clearanceMax = 10.0f;
+};
+struct Area {
+    int ID = -1;
+    float x = 0.0f, y = 0.0f;          // Position
+    float HalfHeight = 0.0f, HalfWidth = 0.0f;
+    SimAreaType Type = NONE;
+    bool Intersects(float px, float py) const {
+        return px <= x + HalfWidth && px >= x - HalfWidth && py <= y + HalfHeight && py >= y - HalfHeight;
+    }
+};
+struct GoalArea : Area { GoalArea() { Type = GOAL; } };
+struct SpawnArea : Area {
+    SpawnArea() { Type = SPAWN; }
+    SpawnConfiguration spawnConfiguration;
+    std::vector<int> connectedGoalAreas;
+    std::vector<float> spawnRate;           // agents per second, per connected goal area
+    std::vector<float> timeSinceLastSpawn;
+};
 
 class Simulator {
    public:
@@ -73,20 +112,128 @@ class Simulator {
     // scene's planner (UpdatePath, :97-124); -1 when the pool is full, the
     // location invalid or no path exists.
     int SpawnAgent(float sx, float sy, float gx, float gy, float clearance, float preferredSpeed) {
-        std::vector<float> path(2 * 4096);
-        int n = 0;
-        int rc = ecm_scene_plan_path(m_Scene, sx, sy, gx, gy, clearance, path.data(), 4096, &n);
-        if (rc != 0 || n < 1) return -1;
-        float st[2] = {sx, sy};
-        int off = 0, id = -1;
-        check(ecm_sim_spawn(m_Sim, 1, st, &clearance, &preferredSpeed, &off, &n, path.data(), 1, &id), "spawn");
+        const int id = SpawnOne(sx, sy, gx, gy, clearance, preferredSpeed, 1);
         m_Dirty = true;
         return id;
     }
     // DestroyAgent (Simulator.cpp:202-208)
     void DestroyAgent(int idx) { check(ecm_sim_destroy_agent(m_Sim, idx), "destroy"); m_Dirty = true; }
-    // Update (Simulator.cpp:314-323)
-    void Update(float /*dt*/) { check(ecm_sim_step(m_Sim, 1), "step"); m_Dirty = true; }
+    // Update (Simulator.cpp:314-323): spawn areas first, as the reference
+    void Update(float /*dt*/) {
+        UpdateSpawnAreas();
+        check(ecm_sim_step(m_Sim, 1), "step");
+        m_Dirty = true;
+    }
+
+    // ValidSpawnLocation (Simulator.cpp:295-311): no active agent closer than
+    // the clearance
+    bool ValidSpawnLocation(float x, float y, float clearance) {
+        sync();
+        return ValidOnMirror(x, y, clearance);
+    }
+
+    // Areas (Simulator.cpp:336-381, 414-470)
+    int AddSpawnArea(float px, float py, float halfW, float halfH, const SpawnConfiguration& config, int ID = -1) {
+        SpawnArea sa;
+        sa.HalfWidth = halfW;
+        sa.HalfHeight = halfH;
+        sa.ID = ID == -1 ? m_NextSpawnID++ : ID;
+        sa.x = px;
+        sa.y = py;
+        sa.spawnConfiguration = config;
+        m_SpawnAreas.emplace(sa.ID, sa);
+        return sa.ID;
+    }
+    int AddGoalArea(float px, float py, float halfW, float halfH, int ID = -1) {
+        GoalArea ga;
+        ga.ID = ID == -1 ? m_NextGoalID++ : ID;
+        ga.x = px;
+        ga.y = py;
+        ga.HalfWidth = halfW;
+        ga.HalfHeight = halfH;
+        m_GoalAreas.emplace(ga.ID, ga);
+        return ga.ID;
+    }
+    void RemoveArea(SimAreaType type, int ID) {
+        if (type == SPAWN) m_SpawnAreas.erase(ID);
+        if (type == GOAL) m_GoalAreas.erase(ID);
+    }
+    void ConnectSpawnGoalAreas(int spawnID, int goalID, float spawnRate = 0.0f) {
+        SpawnArea& sa = m_SpawnAreas[spawnID];
+        for (int ga : sa.connectedGoalAreas)
+            if (ga == goalID) return;
+        sa.connectedGoalAreas.push_back(goalID);
+        sa.timeSinceLastSpawn.push_back(0.0f);
+        sa.spawnRate.push_back(spawnRate);
+    }
+    void DeconnectSpawnGoalAreas(int spawnID, int goalID) {
+        auto it = m_SpawnAreas.find(spawnID);
+        if (it == m_SpawnAreas.end()) return;
+        SpawnArea& sa = it->second;
+        for (size_t i = 0; i < sa.connectedGoalAreas.size(); ++i)
+            if (sa.connectedGoalAreas[i] == goalID) {
+                sa.connectedGoalAreas.erase(sa.connectedGoalAreas.begin() + i);
+                sa.spawnRate.erase(sa.spawnRate.begin() + i);
+                sa.timeSinceLastSpawn.erase(sa.timeSinceLastSpawn.begin() + i);
+                return;
+            }
+    }
+    std::map<int, SpawnArea>& GetSpawnAreas() { return m_SpawnAreas; }
+    std::map<int, GoalArea>& GetGoalAreas() { return m_GoalAreas; }
+    SpawnArea* GetSpawnArea(int ID) {
+        auto it = m_SpawnAreas.find(ID);
+        return it == m_SpawnAreas.end() ? nullptr : &it->second;
+    }
+    GoalArea* GetGoalArea(int ID) {
+        auto it = m_GoalAreas.find(ID);
+        return it == m_GoalAreas.end() ? nullptr : &it->second;
+    }
+    void SetAreaSeed(uint32_t seed) { m_Rng.seed(seed); }
+
+    // UpdateSpawnAreas (Simulator.cpp:494-536): per connection, the time since
+    // the last spawn grows by the step time; int(time * rate) agents spawn,
+    // each at the first of up to 10 random positions in the area where
+    // ValidSpawnLocation holds, with the minimum clearance and preferred
+    // speed of the area's configuration and a random goal in the goal area;
+    // then time -= spawned / rate.
+    void UpdateSpawnAreas() {
+        // the host mirror is read once; agents spawned here are added to its
+        // positions and flags as they are placed
+        bool spawned = false;
+        if (!m_SpawnAreas.empty()) sync();
+        for (auto& kv : m_SpawnAreas) {
+            SpawnArea& area = kv.second;
+            for (size_t ga = 0; ga < area.connectedGoalAreas.size(); ++ga) {
+                area.timeSinceLastSpawn[ga] += m_Params.dt;
+                const float rate = area.spawnRate[ga];
+                auto goal = m_GoalAreas.find(area.connectedGoalAreas[ga]);
+                if (!(rate > 0.0f) || goal == m_GoalAreas.end()) continue;
+                const int agentsToSpawn = (int)(area.timeSinceLastSpawn[ga] * rate);
+                for (int i = 0; i < agentsToSpawn; ++i) {
+                    const float clearance = area.spawnConfiguration.clearanceMin;
+                    const float speed = area.spawnConfiguration.preferredSpeedMin;
+                    float sx = 0.0f, sy = 0.0f;
+                    bool found = false;
+                    for (int attempt = 0; attempt < 10 && !found; ++attempt) {
+                        RandomPositionInArea(area, sx, sy);
+                        found = ValidOnMirror(sx, sy, clearance);
+                    }
+                    if (!found) continue;
+                    float gx = 0.0f, gy = 0.0f;
+                    RandomPositionInArea(goal->second, gx, gy);
+                    const int id = SpawnOne(sx, sy, gx, gy, clearance, speed, 0);
+                    if (id < 0) continue;
+                    m_Positions[id].x = sx;
+                    m_Positions[id].y = sy;
+                    m_Active[id] = 1;
+                    m_LastIdx = id > m_LastIdx ? id : m_LastIdx;
+                    spawned = true;
+                }
+                area.timeSinceLastSpawn[ga] -= (float)agentsToSpawn / rate;
+            }
+        }
+        if (spawned) m_Dirty = true;
+    }
     // Reset (Simulator.cpp:325-333)
     void Reset() { check(ecm_sim_reset(m_Sim), "reset"); m_Dirty = true; }
 
@@ -124,11 +271,40 @@ class Simulator {
         check(ecm_sim_read(m_Sim, ECM_C_ATTRACTION, m_AttractionPoints.data(), N * 8), "read");
         check(ecm_sim_read(m_Sim, ECM_C_CLEARANCE, m_Clearances.data(), N * 4), "read");
         check(ecm_sim_read(m_Sim, ECM_C_ACTIVE, m_Active.data(), N), "read");
+        check(ecm_sim_last_index(m_Sim, &m_LastIdx), "last_index");
         m_Dirty = false;
     }
     void ClearSimulator() {
         if (m_Sim) ecm_sim_destroy(m_Sim);
         m_Sim = nullptr;
+    }
+    // plans the path with the scene's planner, then spawns; -1 when no path
+    // exists, the pool is full or (validate) the location is taken
+    int SpawnOne(float sx, float sy, float gx, float gy, float clearance, float speed, int validate) {
+        std::vector<float> path(2 * 4096);
+        int n = 0;
+        int rc = ecm_scene_plan_path(m_Scene, sx, sy, gx, gy, clearance, path.data(), 4096, &n);
+        if (rc != 0 || n < 1) return -1;
+        float st[2] = {sx, sy};
+        int off = 0, id = -1;
+        check(ecm_sim_spawn(m_Sim, 1, st, &clearance, &speed, &off, &n, path.data(), validate, &id), "spawn");
+        return id;
+    }
+    bool ValidOnMirror(float x, float y, float clearance) const {
+        const float c2 = clearance * clearance;
+        for (int i = 0; i <= m_LastIdx; ++i) {
+            if (!m_Active[i]) continue;
+            const float dx = x - m_Positions[i].x, dy = y - m_Positions[i].y;
+            if (dx * dx + dy * dy < c2) return false;
+        }
+        return true;
+    }
+    // Area::GetRandomPositionInArea (Area.h:39-53), uniform in the rectangle
+    void RandomPositionInArea(const Area& a, float& px, float& py) {
+        const float u = (float)(m_Rng() >> 8) * (1.0f / 16777216.0f);
+        const float v = (float)(m_Rng() >> 8) * (1.0f / 16777216.0f);
+        px = (a.x - a.HalfWidth) + u * (2.0f * a.HalfWidth);
+        py = (a.y - a.HalfHeight) + v * (2.0f * a.HalfHeight);
     }
 
     ecm_scene* m_Scene;
@@ -141,6 +317,11 @@ class Simulator {
     std::vector<VelocityComponent> m_Velocities, m_PreferredVelocities;
     std::vector<ClearanceComponent> m_Clearances;
     std::vector<unsigned char> m_Active;
+    std::map<int, SpawnArea> m_SpawnAreas;
+    std::map<int, GoalArea> m_GoalAreas;
+    int m_NextSpawnID = 0, m_NextGoalID = 0;
+    int m_LastIdx = -1;
+    std::mt19937 m_Rng{12345u};
 };
 
 }  // namespace Simulation

@@ -18,3 +18,15 @@ def test_cpp_facade_drive(native_gpu):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "facade ok" in r.stdout
+
+
+def test_cpp_facade_spawn_goal_areas(native_gpu):
+    """Spawn and goal areas (Simulator.h:92-104, Simulator.cpp:494-536):
+    agents per the time accumulator, inside the spawn area, heading for the
+    goal area; deconnect / remove / missing goal area / zero rate stop it."""
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "areas_drive")
+    assert os.path.exists(exe), "build() compiles tests/cpp/areas_drive.cpp"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "areas ok" in r.stdout
