@@ -17,8 +17,10 @@
 // spawns nothing (the reference default-constructs the goal area with
 // uninitialised extents, and divides 0 by a zero rate); Deconnect removes
 // the connection's rate and timer with it (the reference keeps them, which
-// shifts the later connections' rates); obstacle areas (ECM edits) are not
-// provided.
+// shifts the later connections' rates).  Obstacle areas
+// (Simulator.cpp:383-412) add a rectangle to the scene; the simulator sees
+// it at once and, with updateECM, on the rebuilt ECM (the scene's own
+// builder, not Boost's Voronoi).
 #pragma once
 
 #include <cstdint>
@@ -61,6 +63,10 @@ struct Area {
     }
 };
 struct GoalArea : Area { GoalArea() { Type = GOAL; } };
+struct ObstacleArea : Area {
+    ObstacleArea() { Type = OBSTACLE; }
+    std::vector<float> obstacleVerts;       // x, y pairs, CCW
+};
 struct SpawnArea : Area {
     SpawnArea() { Type = SPAWN; }
     SpawnConfiguration spawnConfiguration;
@@ -88,18 +94,8 @@ class Simulator {
     // obstacles from the scene, replanning through the scene's planner.
     void Initialize() {
         check(ecm_sim_create(&m_Params, m_Box[0], m_Box[1], m_Box[2], m_Box[3], &m_Sim), "create");
-        int nr = 0, nv = 0;
-        check(ecm_scene_obstacle_counts(m_Scene, &nr, &nv), "obstacle_counts");
-        std::vector<float> xy(2 * (size_t)nv + 2);
-        std::vector<int> off(nr + 1);
-        check(ecm_scene_obstacle_export(m_Scene, xy.data(), off.data()), "obstacle_export");
-        check(ecm_sim_set_obstacles(m_Sim, xy.data(), off.data(), nr), "set_obstacles");
-        int ev = 0, ee = 0;
-        check(ecm_scene_ecm_counts(m_Scene, &ev, &ee), "ecm_counts");
-        std::vector<float> vxy(2 * (size_t)ev), pts(8 * (size_t)ee);
-        std::vector<int> tgt(2 * (size_t)ee);
-        check(ecm_scene_ecm_export(m_Scene, vxy.data(), nullptr, tgt.data(), nullptr, pts.data()), "ecm_export");
-        check(ecm_sim_set_ecm(m_Sim, ev, vxy.data(), ee, tgt.data(), pts.data()), "set_ecm");
+        UploadObstacles();
+        UploadEcm();
         check(ecm_sim_set_replan_callback(m_Sim, ecm_scene_replan_callback(), m_Scene), "replan_callback");
         check(ecm_sim_set_replan_threads(m_Sim, 8), "replan_threads");
         const size_t N = (size_t)m_MaxNumEntities;
@@ -190,6 +186,30 @@ class Simulator {
     }
     void SetAreaSeed(uint32_t seed) { m_Rng.seed(seed); }
 
+    // AddObstacleArea (Simulator.cpp:383-412): the rectangle's CCW ring goes
+    // to the scene (Environment::AddObstacle) and to the simulator's
+    // obstacles; updateECM rebuilds the scene's ECM and hands it over.
+    int AddObstacleArea(float px, float py, float halfW, float halfH, bool updateECM = false) {
+        ObstacleArea oa;
+        oa.ID = m_ObstacleAreas.empty() ? 0 : m_ObstacleAreas.back().ID + 1;
+        oa.x = px;
+        oa.y = py;
+        oa.HalfWidth = halfW;
+        oa.HalfHeight = halfH;
+        oa.obstacleVerts = {px + halfW, py + halfH, px - halfW, py + halfH, px - halfW, py - halfH, px + halfW, py - halfH};
+        check(ecm_scene_add_obstacle(m_Scene, oa.obstacleVerts.data(), 4), "add_obstacle");
+        if (m_Sim) {
+            UploadObstacles();
+            if (updateECM) {
+                check(ecm_scene_build_ecm(m_Scene, 0.0f, 0), "build_ecm");
+                UploadEcm();
+            }
+        }
+        m_ObstacleAreas.push_back(oa);
+        return oa.ID;
+    }
+    std::vector<ObstacleArea>& GetObstacleAreas() { return m_ObstacleAreas; }
+
     // UpdateSpawnAreas (Simulator.cpp:494-536): per connection, the time since
     // the last spawn grows by the step time; int(time * rate) agents spawn,
     // each at the first of up to 10 random positions in the area where
@@ -278,6 +298,22 @@ class Simulator {
         if (m_Sim) ecm_sim_destroy(m_Sim);
         m_Sim = nullptr;
     }
+    void UploadObstacles() {
+        int nr = 0, nv = 0;
+        check(ecm_scene_obstacle_counts(m_Scene, &nr, &nv), "obstacle_counts");
+        std::vector<float> xy(2 * (size_t)nv + 2);
+        std::vector<int> off(nr + 1);
+        check(ecm_scene_obstacle_export(m_Scene, xy.data(), off.data()), "obstacle_export");
+        check(ecm_sim_set_obstacles(m_Sim, xy.data(), off.data(), nr), "set_obstacles");
+    }
+    void UploadEcm() {
+        int ev = 0, ee = 0;
+        check(ecm_scene_ecm_counts(m_Scene, &ev, &ee), "ecm_counts");
+        std::vector<float> vxy(2 * (size_t)ev), pts(8 * (size_t)ee);
+        std::vector<int> tgt(2 * (size_t)ee);
+        check(ecm_scene_ecm_export(m_Scene, vxy.data(), nullptr, tgt.data(), nullptr, pts.data()), "ecm_export");
+        check(ecm_sim_set_ecm(m_Sim, ev, vxy.data(), ee, tgt.data(), pts.data()), "set_ecm");
+    }
     // plans the path with the scene's planner, then spawns; -1 when no path
     // exists, the pool is full or (validate) the location is taken
     int SpawnOne(float sx, float sy, float gx, float gy, float clearance, float speed, int validate) {
@@ -319,6 +355,7 @@ class Simulator {
     std::vector<unsigned char> m_Active;
     std::map<int, SpawnArea> m_SpawnAreas;
     std::map<int, GoalArea> m_GoalAreas;
+    std::vector<ObstacleArea> m_ObstacleAreas;
     int m_NextSpawnID = 0, m_NextGoalID = 0;
     int m_LastIdx = -1;
     std::mt19937 m_Rng{12345u};

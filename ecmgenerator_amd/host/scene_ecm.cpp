@@ -424,14 +424,14 @@ int Scene::add_obstacle(const float* xy, int n) {
             ok = (xs[i] == xs[j]) != (ys[i] == ys[j]);
         }
         if (ok) {
+            // the ECM built so far stays in use until the next build_ecm
+            // (Environment::AddObstacle with updateECM false, Environment.cpp:285-315)
             rects.push_back({a, c, b, d});
-            has_ecm = false;
             rgrid_ready = false;
             return 0;
         }
     }
     rect_only = false;
-    has_ecm = false;
     return 0;
 }
 

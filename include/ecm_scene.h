@@ -24,7 +24,9 @@ typedef struct ecm_scene ecm_scene;
 int ecm_scene_create(float xmin, float ymin, float xmax, float ymax, ecm_scene** out);
 void ecm_scene_destroy(ecm_scene* s);
 /* Environment::AddObstacle (Environment.cpp:285-315): CCW ring of n >= 2
- * vertices.  The ECM builder accepts axis-aligned rectangles. */
+ * vertices.  The ECM builder accepts axis-aligned rectangles.  An ECM built
+ * before stays in use (planning, export) until ecm_scene_build_ecm runs
+ * again, as with updateECM false in the reference. */
 int ecm_scene_add_obstacle(ecm_scene* s, const float* xy, int n);
 /* ComputeECM / GenerateECM (Environment.cpp:232-239, ECMGenerator.cpp:235-256).
  * resolution: label-grid spacing in m (<= 0: 0.5); threads <= 0: all cores. */

@@ -103,6 +103,28 @@ int main() {
     for (int f = 0; f < 5; ++f) sim.Update(dt);
     expect(sim.GetNumAgents() == before, "missing goal area or zero rate: no spawns");
 
+    // obstacle area with an ECM update (Simulator.cpp:383-412)
+    int nr0 = 0, nv0 = 0, ev0 = 0, ee0 = 0;
+    ecm_scene_obstacle_counts(scene, &nr0, &nv0);
+    ecm_scene_ecm_counts(scene, &ev0, &ee0);
+    const int oa = sim.AddObstacleArea(0.0f, 12.0f, 3.0f, 2.0f, true);
+    const int ob = sim.AddObstacleArea(-12.0f, 12.0f, 1.0f, 1.0f, false);
+    int nr1 = 0, nv1 = 0, ev1 = 0, ee1 = 0;
+    ecm_scene_obstacle_counts(scene, &nr1, &nv1);
+    ecm_scene_ecm_counts(scene, &ev1, &ee1);
+    expect(oa == 0 && ob == 1 && sim.GetObstacleAreas().size() == 2, "obstacle area ids");
+    expect(nr1 == nr0 + 2 && nv1 == nv0 + 8, "obstacle areas join the scene as 4-vertex rings");
+    expect(ee1 != ee0, "the ECM was rebuilt with the first obstacle area");
+    expect(ecm_scene_inside_obstacle(scene, 0.0f, 12.0f) == 1 && ecm_scene_inside_obstacle(scene, 0.0f, 16.0f) == 0,
+           "the new obstacle covers its rectangle");
+    int bad = 0;
+    for (int f = 0; f < 20; ++f) sim.Update(dt);
+    PositionComponent* p = sim.GetPositionData();
+    act = sim.GetActiveFlags();
+    for (int i = 0; i <= sim.GetLastIndex(); ++i)
+        if (act[i] && !(std::isfinite(p[i].x) && std::isfinite(p[i].y))) ++bad;
+    expect(bad == 0, "the simulation steps on after the obstacle areas");
+
     std::printf(fails ? "areas FAILED\n" : "areas ok\n");
     ecm_scene_destroy(scene);
     return fails ? 1 : 0;
