@@ -24,6 +24,8 @@
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_unsort(const DevState& s, hipStream_t st);
+hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
+                            const unsigned long long* acc, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
@@ -172,6 +174,13 @@ struct ecm_sim {
     bool id_current = true;                     // the slot-indexed arrays hold it
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
     DBuf<int> fwd, prev_of, pos_of;    // previous <-> this step's sorted positions; id -> last one
+    // host changes kept out of the slot-entry build: agents spawned and
+    // agents removed since the last step (uploaded before the next one)
+    std::vector<int> pending_new, pending_dead, new_stage, dead_stage;
+    std::vector<uint8_t> h_pending;
+    unsigned long long n_new_stage = 0;
+    DBuf<int> new_ids, dead_ids;
+    int last_grid_n = 0;               // agents in the last step's grid
 
     // Per-step counters and the event list share one device block, so one
     // copy brings both back: words [0..7] are the counters ([0] events,
@@ -300,6 +309,7 @@ static void free_all(ecm_sim* s) {
     s->key.release(); s->rank.release(); s->cell_count.release(); s->cell_start.release(); s->bsum.release();
     for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); }
     s->fwd.release(); s->prev_of.release(); s->pos_of.release();
+    s->new_ids.release(); s->dead_ids.release();
     s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
     for (auto& e : s->envs) e->release();
@@ -413,6 +423,9 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc) rc |= s->fwd.alloc(N);
     if (!rc) rc |= s->prev_of.alloc(N);
     if (!rc) rc |= s->pos_of.alloc(N);
+    if (!rc) rc |= s->new_ids.alloc(N);
+    if (!rc) rc |= s->dead_ids.alloc(N);
+    if (!rc) s->h_pending.assign(N, 0);
     if (!rc) rc |= s->xfer.alloc(8 + 4 * N);
     if (!rc) rc |= s->wstats.alloc(256);
     if (!rc) rc |= s->acc.alloc(16);
@@ -928,7 +941,14 @@ extern "C" int ecm_sim_env_spawn(ecm_sim* s, int env, int n, const float* start_
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     d_ids.release(); d_off.release(); d_cnt.release(); d_st.release(); d_r.release(); d_s.release();
-    s->sorted_valid = false;                    // the next step starts from the slot arrays
+    // the next step keeps entering from the last step's sorted outputs and
+    // appends the new agents (read from the slot arrays)
+    if (s->sorted_valid)
+        for (int i = 0; i < n; ++i)
+            if (ids[i] >= 0 && !s->h_pending[ids[i]]) {
+                s->h_pending[ids[i]] = 1;
+                s->pending_new.push_back(ids[i]);
+            }
     return rc;
 }
 
@@ -936,6 +956,47 @@ extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const flo
                              const int* path_off, const int* path_cnt, const float* path_xy, int validate,
                              int* out_ids) {
     return ecm_sim_env_spawn(s, 0, n, start_xy, radius, pref_speed, path_off, path_cnt, path_xy, validate, out_ids);
+}
+
+// The next step enters from the slot arrays: host changes pending for a
+// sorted-entry build are dropped (the slot arrays already hold them).
+static void invalidate_sorted(ecm_sim* s) {
+    s->sorted_valid = false;
+    for (int id : s->pending_new) s->h_pending[id] = 0;
+    s->pending_new.clear();
+    s->pending_dead.clear();
+}
+
+// Before a step that enters from the last step's sorted outputs: agents the
+// host removed are marked destroyed at their last sorted position, and the
+// ids of agents spawned since then go to the device with their count
+// (acc[15]).  Falls back to the slot-entry build when the appended agents
+// would not fit the grid build's n_slots elements.
+static int upload_pending(ecm_sim* s) {
+    if (!s->sorted_valid) { invalidate_sorted(s); return 0; }
+    if ((size_t)s->last_grid_n + s->pending_new.size() > (size_t)s->prm.max_agents) {
+        invalidate_sorted(s);
+        return 0;
+    }
+    const int o = s->cur ^ 1;
+    if (!s->pending_dead.empty()) {
+        s->dead_stage.swap(s->pending_dead);
+        s->pending_dead.clear();
+        const int n = (int)s->dead_stage.size();
+        HIPCHK(hipMemcpyAsync(s->dead_ids.p, s->dead_stage.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(launch_mark_dead(s->dead_ids.p, n, s->pos_of.p, s->sorted_pr[o].p, s->dead_s[o].p, s->acc.p, s->stream));
+    }
+    if (!s->pending_new.empty()) {
+        s->new_stage.swap(s->pending_new);
+        s->pending_new.clear();
+        for (int id : s->new_stage) s->h_pending[id] = 0;
+        s->n_new_stage = s->new_stage.size();
+        HIPCHK(hipMemcpyAsync(s->new_ids.p, s->new_stage.data(), s->new_stage.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->acc.p + 15, &s->n_new_stage, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                              s->stream));
+    }
+    return 0;
 }
 
 static void mark_destroyed(ecm_sim* s, int id) {
@@ -955,7 +1016,17 @@ extern "C" int ecm_sim_destroy_agent(ecm_sim* s, int id) {
     HIPCHK(hipMemcpyAsync(s->active.p + id, &z, 1, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     mark_destroyed(s, id);
-    s->sorted_valid = false;                    // the next step starts from the slot arrays
+    // an agent spawned since the last step leaves the list of new ones; any
+    // other leaves the last step's sorted order (marked destroyed there
+    // before the next step)
+    if (s->sorted_valid) {
+        if (s->h_pending[id]) {
+            s->h_pending[id] = 0;
+            s->pending_new.erase(std::find(s->pending_new.begin(), s->pending_new.end(), id));
+        } else {
+            s->pending_dead.push_back(id);
+        }
+    }
     return 0;
 }
 
@@ -1138,7 +1209,7 @@ static DevState make_state(ecm_sim* s) {
     d.sorted_pr = s->sorted_pr[s->cur].p; d.sorted_v = s->sorted_v[s->cur].p;
     d.nbr = s->nbr[s->cur].p; d.nbr_cnt = s->nbr_cnt[s->cur].p;
     d.nbr_prev = s->nbr[s->cur ^ 1].p; d.nbr_cnt_prev = s->nbr_cnt[s->cur ^ 1].p;
-    d.fwd = s->fwd.p; d.prev_of = s->prev_of.p; d.pos_of = s->pos_of.p;
+    d.fwd = s->fwd.p; d.prev_of = s->prev_of.p; d.pos_of = s->pos_of.p; d.new_ids = s->new_ids.p;
     d.events = reinterpret_cast<int4*>(s->xfer.p + 8); d.events_n = s->xfer.p;
     d.stats = s->xfer.p + 2;
     d.wstats = s->wstats.p;
@@ -1277,6 +1348,7 @@ static int prepare_step(ecm_sim* s, DevState* out) {
     while (s->last_idx >= 0 && !s->h_active[s->last_idx]) --s->last_idx;
     int rc = ensure_agent_grid(s);
     if (rc) return rc;
+    if ((rc = upload_pending(s)) != 0) return rc;
     {
         const float range = obstacle_range(s);
         int maxo = 16;
@@ -1480,6 +1552,7 @@ static int step_once(ecm_sim* s) {
     s->sorted_valid = true;                     // the state now lives in this step's sorted outputs
     s->id_current = false;
     const int n_before = s->num_agents;
+    s->last_grid_n = n_before;                // this step's grid held every agent active at its start
     std::sort(dead.begin(), dead.end());      // reference visits slots in ascending order
     for (int id : dead) mark_destroyed(s, id);
     s->stat[0] += 1;
@@ -1673,7 +1746,7 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
             else s->free_stack.push_back(i);
         }
     }
-    s->sorted_valid = false;                    // the next step starts from the slot arrays
+    invalidate_sorted(s);                       // the next step starts from the slot arrays
     return 0;
 }
 

@@ -115,6 +115,8 @@ struct DevState {
     int* fwd;                 // [n_slots] previous sorted position -> this step's, -1: left the grid
     int* prev_of;             // [n_slots] this step's sorted position -> previous, -1: none
     int* pos_of;              // [n_slots] agent id -> its sorted position in the last step (k_unsort)
+    const int* new_ids;       // agents spawned since the last step, appended to a sorted-entry build
+                              // (their count in acc[15], cleared by k_attr_knn)
 
     // ---- events ----
     // One list per step, read back by the host after the neighbour pass:

@@ -106,14 +106,26 @@ __global__ __launch_bounds__(256) void k_grid_count(DevState s) {
 // leave the grid.
 // The last step's active count is acc[13] (its scan); it is kept in
 // acc[14] for the scatter, since this step's scan overwrites acc[13].
+// Agents spawned since the last step (acc[15] of them, ids in new_ids) follow
+// as elements n_old, n_old + 1, ... read from the slot arrays; the host
+// keeps n_old + their count within n_slots.
 __global__ __launch_bounds__(256) void k_grid_count_sorted(DevState s) {
     const uint32_t n_old = (uint32_t)s.acc[13];
+    const uint32_t n_new = (uint32_t)s.acc[15];
     if (blockIdx.x == 0) {
         step_begin(s.events_n, s.wstats, s.acc);
         if (threadIdx.x == 0) s.acc[14] = n_old;
     }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int)n_old) return;
+    if (t >= (int)n_old) {
+        if (t >= (int)(n_old + n_new)) return;
+        const int id = s.new_ids[t - n_old];
+        if (!s.active[id]) { s.key[t] = kInvalidKey; return; }
+        const uint32_t m = cell_key(s.envs[slot_env(s, id)], s.pos_in[id]);
+        s.key[t] = m;
+        s.rank[t] = atomicAdd(&s.cell_count[m], 1u);
+        return;
+    }
     if (s.dead_s_old[t]) { s.key[t] = kInvalidKey; s.fwd[t] = -1; return; }
     const uint32_t m = cell_key(env_of_sorted(s, s.sorted_v_old[t]), s.pos_s_old[t]);
     s.key[t] = m;
@@ -217,10 +229,20 @@ __global__ void k_grid_scatter(DevState s) {
 // seeds.
 __global__ void k_grid_scatter_sorted(DevState s) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int)(uint32_t)s.acc[14]) return;
+    const int n_old = (int)(uint32_t)s.acc[14];
+    if (t >= n_old + (int)(uint32_t)s.acc[15]) return;
     const uint32_t m = s.key[t];
     if (m == kInvalidKey) return;
     const uint32_t slot = s.cell_start[m] + s.rank[t];
+    if (t >= n_old) {                                  // spawned since the last step: no seeds
+        const int id = s.new_ids[t - n_old];
+        const float2 p = s.pos_in[id], v = s.vel_in[id];
+        s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[id], __int_as_float(id));
+        s.sorted_v[slot] = make_float4(v.x, v.y, s.pref_speed[id], __int_as_float(slot_env(s, id)));
+        s.attr_s[slot] = s.attraction[id];
+        s.prev_of[slot] = -1;
+        return;
+    }
     const float2 p = s.pos_s_old[t], v = s.vel_s_old[t];
     const float4 pr = s.sorted_pr_old[t], sv = s.sorted_v_old[t];
     s.sorted_pr[slot] = make_float4(p.x, p.y, pr.z, pr.w);
@@ -228,6 +250,18 @@ __global__ void k_grid_scatter_sorted(DevState s) {
     s.attr_s[slot] = s.attr_s_old[t];
     s.fwd[t] = (int)slot;
     s.prev_of[slot] = t;
+}
+
+// Agents the host removed since the last step leave a sorted-entry build:
+// their last sorted position (pos_of, checked against the id there) is
+// marked destroyed.
+__global__ void k_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
+                            const unsigned long long* acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int id = ids[i];
+    const int u = pos_of[id];
+    if (u >= 0 && u < (int)(uint32_t)acc[13] && __float_as_int(sorted_pr_old[u].w) == id) dead_old[u] = 1;
 }
 
 // Brings the slot-indexed arrays up to date from the last step's sorted
@@ -423,14 +457,15 @@ __device__ __forceinline__ void nb_insert(const DevState& s, float (&d)[K], int 
 // had as neighbours in the previous step, taken to this step's sorted
 // positions through fwd, when all of them are still on the grid of its
 // environment and none is the agent itself (any k distinct candidates bound
-// the k-th distance from above; fwd is one-to-one).  INFINITY when there is
-// no such seed.  The seeds (this step's sorted positions) and their squared
-// distances are left in ss / sd, unordered.
+// the k-th distance from above; fwd is one-to-one).  INFINITY when there
+// is no such seed (spawned since the last step, or a neighbour left).  The
+// seeds (this step's sorted positions) and their squared distances are left
+// in ss / sd, unordered.
 template <int K>   // K == k
-__device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, int id, V2 p, float (&sd)[K], int (&ss)[K]) {
+__device__ __forceinline__ float knn_seed_bound(const DevState& s, int t, V2 p, float (&sd)[K], int (&ss)[K]) {
     const int q = s.prev_of[t];
-    if (q < 0 || q >= s.n_slots || s.nbr_cnt_prev[q] < K) return INFINITY;
     const int env = s.agent_env ? __float_as_int(s.sorted_v[t].w) : 0;
+    if (q < 0 || q >= s.n_slots || s.nbr_cnt_prev[q] < K) return INFINITY;
     int* ns = ss;
     bool ok = true;
 #pragma unroll
@@ -694,6 +729,9 @@ __device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc
 // each) for the seeded walk; (kKnnBuf + 2 kKnnRanges) * blockDim.x ints per
 // block, lane-interleaved.
 static constexpr int kKnnBuf = 16;
+// Seeded agents walk their bound's cell rectangle when its radius is at most
+// this many cells.
+static constexpr float kKnnWalkCells = 6.0f;
 static constexpr int kKnnRanges = 4;
 
 // One sorted agent: attraction point, steering, k nearest.  Returns whether
@@ -763,9 +801,13 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     // k nearest (replaces KDTree::KNearestAgents)
     int slot[K];
     float sd[K];
-    const float bound = knn_seed_bound<K>(s, t, id, pos, sd, slot);
+    const float bound = knn_seed_bound<K>(s, t, pos, sd, slot);
     int* buf = knn_lds + threadIdx.x;
-    const int cnt = bound < INFINITY
+    // the one-pass walk covers the bound's whole cell rectangle: a loose
+    // bound (seeds from far away, a snapshot from another time) goes to the
+    // ring search instead, which stops at the k-th candidate
+    const float wmax = kKnnWalkCells * E.ag.cell;
+    const int cnt = bound <= wmax * wmax
         ? knn_query_seeded<K, kKnnBuf, kKnnRanges>(s, E, pos, bound, buf, buf + kKnnBuf * blockDim.x, blockDim.x,
                                                    sd, slot)
         : knn_query<K, kKnnBuf>(s, E, pos, bound, buf, blockDim.x, slot);
@@ -809,6 +851,7 @@ __device__ __forceinline__ void publish_if_last(const DevState& s) {
 template <int K>
 __global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
     extern __shared__ int knn_lds[];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.acc[15] = 0;   // the grid build has used it
     attr_knn_agent<K>(s, knn_lds);
     publish_if_last(s);
 }
@@ -1562,6 +1605,13 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st) 
     hipLaunchKernelGGL(k_scan_add, dim3(cdiv(G, 256)), dim3(256), 0, st, s.cell_start, (const uint32_t*)bsum, G);
     if (s.sorted_in) hipLaunchKernelGGL(k_grid_scatter_sorted, ga, b, 0, st, s);
     else hipLaunchKernelGGL(k_grid_scatter, ga, b, 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
+                            const unsigned long long* acc, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_mark_dead, dim3(cdiv(n, 256)), dim3(256), 0, st, ids, n, pos_of, sorted_pr_old,
+                                  dead_old, acc);
     return hipGetLastError();
 }
 
