@@ -109,3 +109,33 @@ def test_slots_reused_across_environments():
             print(f"step {it} env {e}: {res}", flush=True)
             assert all(v == 0 for v in res.values()), (it, e, res)
     sim.close()
+
+
+def test_spawn_remove_respawn_between_steps():
+    """Between two steps: spawn a batch, remove some of the agents just
+    spawned and some older ones, then spawn again into the freed slots (LIFO:
+    the same ids).  The next step appends the spawned agents to the grid
+    build and marks the removed ones at their last sorted positions; every
+    step must still equal the oracle bit for bit."""
+    box, _, obs, ecm, agents = small_scene(47, half=30.0, n_rects=4, n_agents=1600)
+    p = Params(max_agents=2048, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    sim.SpawnAgents(subset(agents, np.arange(0, 1000)))
+    for _ in range(4):
+        sim.Update()
+    a = sim.SpawnAgents(subset(agents, np.arange(1000, 1300)))
+    a = a[a >= 0]
+    for i in a[::4]:
+        sim.DestroyAgent(int(i))                   # spawned since the last step
+    live = np.flatnonzero(sim.state()["active"].astype(bool))
+    old = np.setdiff1d(live, a)
+    for i in old[::7]:
+        sim.DestroyAgent(int(i))                   # in the last step's order
+    b = sim.SpawnAgents(subset(agents, np.arange(1300, 1600)))
+    b = b[b >= 0]
+    reused = np.intersect1d(b, np.concatenate([a[::4], old[::7]]))
+    print(f"spawned {a.size} + {b.size}, {reused.size} into freed slots", flush=True)
+    assert reused.size > 50
+    for it in range(3):
+        check_step(sim, p, obs, ecm, f"after spawn/remove/respawn step {it}")
+    sim.close()
