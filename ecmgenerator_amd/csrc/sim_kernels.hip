@@ -697,20 +697,28 @@ __device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc
         if (nr > 0) {
             int r = 0;
             uint32_t j = (uint32_t)rbuf[0], e = (uint32_t)rbuf[bstride];
-            while (true) {
-                const float2 c = *reinterpret_cast<const float2*>(&s.sorted_pr[j]);
+            auto take = [&](uint32_t jj, float2 c) {
                 const float dx = c.x - p.x, dy = c.y - p.y;
                 const float d2 = dx * dx + dy * dy;
                 if (d2 > kEps && d2 <= thr) {
                     bool listed = false;
 #pragma unroll
-                    for (int i = 0; i < K; ++i) listed |= sl[i] == (int)j;
+                    for (int i = 0; i < K; ++i) listed |= sl[i] == (int)jj;
                     if (!listed) {
-                        buf[nb * bstride] = (int)j;
+                        buf[nb * bstride] = (int)jj;
                         if (++nb == B) knn_merge<K>(s, p, buf, bstride, nb, d, sl, n, thr, prune2);
                     }
                 }
-                if (++j == e) {
+            };
+            while (true) {
+                // two candidates of the range per trip, their loads together
+                const bool two = j + 1u < e;
+                const float2 c0 = *reinterpret_cast<const float2*>(&s.sorted_pr[j]);
+                const float2 c1 = *reinterpret_cast<const float2*>(&s.sorted_pr[two ? j + 1u : j]);
+                take(j, c0);
+                if (two) take(j + 1u, c1);
+                j += two ? 2u : 1u;
+                if (j == e) {
                     if (++r == nr) break;
                     j = (uint32_t)rbuf[(2 * r) * bstride];
                     e = (uint32_t)rbuf[(2 * r + 1) * bstride];
