@@ -666,23 +666,35 @@ __device__ __forceinline__ int knn_query_seeded(const DevState& s, const EnvDesc
     uint32_t ca = 0, cb = 0;             // open range [ca, cb), empty when ca == cb
     while (true) {
         int nr = 0;
-        while (walking && nr < R) {
-            if (band_d2(p.x, g.x0, g.cell, x, g.side) + dy2 <= prune2) {
-                const uint32_t m = E.cell_base + (mx | my);
-                const uint32_t a = s.cell_start[m], b = s.cell_start[m + 1];
-                if (a < b) {
-                    if (a == cb) cb = b;
-                    else {
-                        if (ca < cb) {
-                            rbuf[(2 * nr) * bstride] = (int)ca;
-                            rbuf[(2 * nr + 1) * bstride] = (int)cb;
-                            ++nr;
-                        }
-                        ca = a; cb = b;
+        auto add = [&](uint32_t a, uint32_t b) {
+            if (a < b) {
+                if (a == cb) cb = b;
+                else {
+                    if (ca < cb) {
+                        rbuf[(2 * nr) * bstride] = (int)ca;
+                        rbuf[(2 * nr + 1) * bstride] = (int)cb;
+                        ++nr;
                     }
+                    ca = a; cb = b;
                 }
             }
-            if (x < xh) { ++x; mx = ((mx | 0xaaaaaaaau) + 1u) & 0x55555555u; }
+        };
+        while (walking && nr < R) {
+            // this cell and, when the row has one, the next: the range loads
+            // of both go out together
+            const bool two = x < xh;
+            const uint32_t mxn = ((mx | 0xaaaaaaaau) + 1u) & 0x55555555u;
+            const uint32_t m0 = E.cell_base + (mx | my);
+            const uint32_t m1 = two ? E.cell_base + (mxn | my) : m0;
+            const bool in0 = band_d2(p.x, g.x0, g.cell, x, g.side) + dy2 <= prune2;
+            const bool in1 = two && band_d2(p.x, g.x0, g.cell, x + 1, g.side) + dy2 <= prune2;
+            const uint32_t a0 = s.cell_start[m0], b0 = s.cell_start[m0 + 1];
+            const uint32_t a1 = s.cell_start[m1], b1 = s.cell_start[m1 + 1];
+            if (in0) add(a0, b0);
+            const bool took_two = two && nr < R;
+            if (took_two && in1) add(a1, b1);
+            const int nx = x + (took_two ? 2 : 1);
+            if (nx <= xh) { x = nx; mx = took_two ? ((mxn | 0xaaaaaaaau) + 1u) & 0x55555555u : mxn; }
             else if (y < yh) {
                 ++y; x = xl; mx = mx0; my = part1by1((uint32_t)y) << 1;
                 dy2 = band_d2(p.y, g.y0, g.cell, y, g.side);
