@@ -127,7 +127,9 @@ __global__ __launch_bounds__(256) void k_grid_count_sorted(DevState s) {
         return;
     }
     if (s.dead_s_old[t]) { s.key[t] = kInvalidKey; s.fwd[t] = -1; return; }
-    const uint32_t m = cell_key(env_of_sorted(s, s.sorted_v_old[t]), s.pos_s_old[t]);
+    // one environment: no need to read the agent's environment
+    const EnvDesc& E = s.agent_env ? env_of_sorted(s, s.sorted_v_old[t]) : s.envs[0];
+    const uint32_t m = cell_key(E, s.pos_s_old[t]);
     s.key[t] = m;
     s.rank[t] = atomicAdd(&s.cell_count[m], 1u);
 }
