@@ -117,21 +117,38 @@ __global__ __launch_bounds__(256) void k_grid_count_sorted(DevState s) {
         if (threadIdx.x == 0) s.acc[14] = n_old;
     }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int)n_old) {
-        if (t >= (int)(n_old + n_new)) return;
+    uint32_t m = kInvalidKey;
+    if (t < (int)n_old) {
+        if (s.dead_s_old[t]) { s.key[t] = kInvalidKey; s.fwd[t] = -1; }
+        else {
+            // one environment: no need to read the agent's environment
+            const EnvDesc& E = s.agent_env ? env_of_sorted(s, s.sorted_v_old[t]) : s.envs[0];
+            m = cell_key(E, s.pos_s_old[t]);
+        }
+    } else if (t < (int)(n_old + n_new)) {
         const int id = s.new_ids[t - n_old];
-        if (!s.active[id]) { s.key[t] = kInvalidKey; return; }
-        const uint32_t m = cell_key(s.envs[slot_env(s, id)], s.pos_in[id]);
-        s.key[t] = m;
-        s.rank[t] = atomicAdd(&s.cell_count[m], 1u);
-        return;
+        if (!s.active[id]) s.key[t] = kInvalidKey;
+        else m = cell_key(s.envs[slot_env(s, id)], s.pos_in[id]);
     }
-    if (s.dead_s_old[t]) { s.key[t] = kInvalidKey; s.fwd[t] = -1; return; }
-    // one environment: no need to read the agent's environment
-    const EnvDesc& E = s.agent_env ? env_of_sorted(s, s.sorted_v_old[t]) : s.envs[0];
-    const uint32_t m = cell_key(E, s.pos_s_old[t]);
-    s.key[t] = m;
-    s.rank[t] = atomicAdd(&s.cell_count[m], 1u);
+    // Lanes in Morton order mostly share cells with their neighbours: each
+    // run of equal keys in consecutive lanes takes one atomic, and its lanes
+    // take consecutive arrival orders from the returned base.  Every lane of
+    // the wavefront takes part (no lane has returned).
+    const int lane = (int)__lane_id();
+    const uint32_t mp = __shfl_up(m, 1u);
+    const bool start = lane == 0 || m != mp;
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long upto = (2ull << lane) - 1ull;          // lanes 0..lane
+    const int leader = 63 - __clzll((long long)(starts & upto));
+    const unsigned long long later = starts & ~upto;
+    const int next = later ? __ffsll((long long)later) - 1 : 64;
+    uint32_t base = 0;
+    if (start && m != kInvalidKey) base = atomicAdd(&s.cell_count[m], (uint32_t)(next - lane));
+    base = __shfl(base, leader);
+    if (m != kInvalidKey) {
+        s.key[t] = m;
+        s.rank[t] = base + (uint32_t)(lane - leader);
+    }
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
