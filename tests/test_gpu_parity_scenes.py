@@ -1,7 +1,9 @@
 """GPU step vs CPU oracle on the seeded scenarios with a built ECM and
 planner paths: bit-exact per step from identical states.  c0 (5k agents) is
-compared in full at several steps; c1 (100k) and c2 (1M) on random agent
-subsets (every agent reads only step-start state, so a subset is exact)."""
+compared in full at several steps and on agent subsets at every step of its
+600; c1 (100k) and c2 (1M) on random agent subsets (every agent reads only
+step-start state, so a subset is exact) at steps 0, 1, 10, 100 and the last
+of each run (600 and 300 steps)."""
 import numpy as np
 import pytest
 
@@ -12,7 +14,7 @@ from helpers import compare_step, oracle_state
 pytestmark = pytest.mark.gpu
 
 
-def run(wl, steps, subset_n=None, keep=True, seed=0):
+def run(wl, steps, subset_n=None, keep=True, seed=0, every=1):
     from oracle import oracle_py as O
     p = wl.params
     p.keep_orca_velocity = keep
@@ -31,7 +33,8 @@ def run(wl, steps, subset_n=None, keep=True, seed=0):
             ora = O.step(p, before, wl.obstacles, wl.ecm, subset=sub)
             sim.Update()
             res = compare_step(sim, before, ora, subset=sub)
-            print(f"{wl.name} step={it} n={int(before['active'].sum())} {res}", flush=True)
+            if it % every == 0 or any(res.values()):
+                print(f"{wl.name} step={it} n={int(before['active'].sum())} {res}", flush=True)
             for k, v in res.items():
                 worst[k] = max(worst.get(k, 0), v)
         else:
@@ -48,6 +51,14 @@ def test_c0_full_parity():
     assert all(v == 0 for v in worst.values()), worst
 
 
+def test_c0_every_step_subset_parity():
+    """All 600 steps of the c0 run, each compared on a fresh random subset
+    of 1,000 agents (about 120 comparisons per agent over the run)."""
+    wl = W.build("c0", seed=1)
+    worst, st = run(wl, steps=range(600), subset_n=1000, seed=3, every=100)
+    assert all(v == 0 for v in worst.values()), worst
+
+
 def test_c0_reference_literal_parity():
     wl = W.build("c0-ref", seed=2)
     worst, st = run(wl, steps=(0, 3, 30))
@@ -56,13 +67,13 @@ def test_c0_reference_literal_parity():
 
 def test_c1_subset_parity():
     wl = W.build("c1", seed=1)
-    worst, st = run(wl, steps=(0, 10), subset_n=1500)
+    worst, st = run(wl, steps=(0, 1, 10, 100, 599), subset_n=1500)
     assert all(v == 0 for v in worst.values()), worst
 
 
 def test_c2_subset_parity_full_size():
     wl = W.build("c2", seed=1)
-    worst, st = run(wl, steps=(0, 8), subset_n=600, keep=False)
+    worst, st = run(wl, steps=(0, 1, 10, 100, 299), subset_n=600, keep=False)
     assert all(v == 0 for v in worst.values()), worst
 
 
