@@ -290,19 +290,26 @@ def cpu_baseline(sim, wl, n_sample):
     rng = np.random.default_rng(0)
     sample = np.sort(rng.choice(act, size=min(n_sample, act.size), replace=False)).astype(np.int32)
     P = O.Problem(wl.params, st, wl.obstacles, wl.ecm, O.KNN_REFERENCE)
-    t0 = time.perf_counter()
-    O.lib().oracle_kdtree_build_only(P.st)
-    t_build = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.step(wl.params, st, wl.obstacles, wl.ecm, subset=sample, knn_mode=O.KNN_REFERENCE)
-    t_sample = time.perf_counter() - t0
+    # both timed legs run pinned to one of the cores this process may use
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        O.lib().oracle_kdtree_build_only(P.st)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.step(wl.params, st, wl.obstacles, wl.ecm, subset=sample, knn_mode=O.KNN_REFERENCE)
+        t_sample = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, allowed)
     # the subset step includes one more tree build; remove it
     per_agent = max(t_sample - t_build, 1e-9) / sample.size
     step_s = t_build + per_agent * act.size
     return {"value": round(act.size / step_s, 2), "unit": "agent-updates/s", "cores": 1, "kind": "port",
             "sample": f"KD-tree build over all {act.size} agents ({t_build:.3f}s) + full update of "
                       f"{sample.size} random agents ({per_agent * 1e3:.3f} ms/agent), extrapolated to one step",
-            "cpu": _cpu_model()}
+            "cpu": _cpu_model(), "pinned_core": core, "host_cpus": len(allowed)}
 
 
 def _cpu_model():
