@@ -7,7 +7,9 @@ step vs CPU oracle bit for bit (NaN equal to NaN):
   steps on from NaN positions.
 * two agents at the same position: d2 = 0 <= 1e-4, so each is left out of
   the other's neighbour list (KDTree.cpp:112) and they pass through each
-  other."""
+  other.
+* an agent between two parallel walls: its two obstacle half-planes have
+  opposite normals, so the 2D LP meets |det| <= 1e-4 (ORCA.cpp:526)."""
 import numpy as np
 import pytest
 
@@ -20,8 +22,8 @@ pytestmark = pytest.mark.gpu
 BOX = (-50.0, -50.0, 50.0, 50.0)
 
 
-def make_sim(starts, goals, vel):
-    obs = S.rects_to_obstacles([(-45.0, 38.0, -38.0, 45.0)])
+def make_sim(starts, goals, vel, rects=((-45.0, 38.0, -38.0, 45.0),)):
+    obs = S.rects_to_obstacles(list(rects))
     ecm = S.empty_square_ecm(*BOX)
     n = len(starts)
     p = Params(max_agents=16)
@@ -77,5 +79,31 @@ def test_coincident_pair_is_invisible_to_itself():
                 nbr, cnt = sim.neighbors()
                 assert ids[1] not in nbr[ids[0], :cnt[ids[0]]] and ids[0] not in nbr[ids[1], :cnt[ids[1]]]
                 assert cnt[ids[0]] == 2 and cnt[ids[1]] == 2
+    finally:
+        sim.close()
+
+
+def test_corridor_parallel_walls():
+    from oracle import oracle_py as O
+    sim, obs, ecm, ids = make_sim(
+        starts=[(0.0, 0.0), (3.0, 0.3), (-3.0, -0.4)],
+        goals=[(9.0, 0.0), (-9.0, 0.3), (9.0, -0.4)],
+        vel=[(1.0, 0.0), (-1.0, 0.0), (0.8, 0.0)],
+        rects=((-10.0, 1.5, 10.0, 5.0), (-10.0, -5.0, 10.0, -1.5)))
+    try:
+        # premise, from the oracle's constraint builder (rows n.x, n.y, p.x, p.y):
+        # the first agent's two wall half-planes are parallel
+        st0 = oracle_state(sim)
+        con, n_obst = O.constraints(sim.params, st0, obs, int(ids[0]), ids[1:])
+        assert n_obst == 2
+        (ax, ay), (bx, by) = con[0, :2], con[1, :2]
+        assert abs(ay * bx - ax * by) <= 1e-4
+        for it in range(5):
+            before = oracle_state(sim)
+            ora = O.step(sim.params, before, obs, ecm)
+            sim.Update()
+            res = compare_step(sim, before, ora)
+            print(f"step={it} {res}", flush=True)
+            assert all(v == 0 for v in res.values()), res
     finally:
         sim.close()
