@@ -65,8 +65,8 @@ def copy_calibration(mb: int) -> None:
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
-              "and not built. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: runs on one GPU only; sharding the step over several GPUs is not implemented. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from ecmgenerator_amd import workloads as W
     from ecmgenerator_amd.simulator import Simulator
@@ -134,7 +134,9 @@ def main():
     names = ["grid_build", "attraction_knn", "orca_lp_integrate"]
     dom = int(np.argmax(kt_mean))
     n_per_step = updates / args.steps
-    # algorithmic HBM bytes per agent-update of each kernel
+    # algorithmic HBM bytes per agent-update of each kernel: the state it
+    # must read and write once per agent (neighbour gathers served from
+    # cache and the grid's own scratch traffic are not counted)
     bytes_per_agent = {
         "grid_build": 1 + 8 + 8 + 4 + 4 + 4 + 16 + 8 + 4,
         "attraction_knn": 16 + 8 + 8 + 8 * mean_path + 8 + 4 + 8 + 8 + 4 * wl.params.k + 4,

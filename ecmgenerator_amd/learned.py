@@ -4,7 +4,7 @@ collection from simulator runs, GPU training and weight files.
 The network predicts which ORCA constraints are active at the LP optimum;
 the step verifies every proposal exactly (restricted LP + containment of
 all other constraints) and falls back to the plain 2D / 3D LP
-(ORCA.cpp:428-669) when the check fails, so the plain LP stays the judge.
+(ORCA.cpp:428-669) when the check fails, so the plain LP has the last word.
 
     python -m ecmgenerator_amd.learned train --out weights.npz
 

@@ -5,7 +5,7 @@
  *
  * The reference has no learned component; the plain incremental LP it
  * proposes for (ORCA::RandomizedLP, ORCA.cpp:428-587, and
- * ORCA::RandomizedLP3D, ORCA.cpp:592-669) stays the judge: an agent's
+ * ORCA::RandomizedLP3D, ORCA.cpp:592-669) has the last word: an agent's
  * proposal is accepted only when the LP restricted to the predicted
  * constraints returns a velocity that every other constraint contains
  * (Constraint::Contains, ORCA.h:49-60), which makes it the optimum of the
