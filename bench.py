@@ -44,7 +44,34 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of replaying step graphs")
     ap.add_argument("--autotune", type=int, default=3,
                     help="steps per autotuner trial during warmup (0: default launch shape)")
+    ap.add_argument("--prerun", type=int, default=None,
+                    help="untimed steps before warmup (default: the workload's own, e.g. until the c2 jam forms)")
     return ap.parse_args()
+
+
+def local_density(pos, active, cell=1.0):
+    """Agents per m^2 on a grid of `cell` m squares: the fullest square,
+    and the 99th percentile and mean of the density around each agent (the
+    count of its own square)."""
+    p = pos[active.astype(bool)].astype(np.float64)
+    if p.shape[0] == 0:
+        return {"peak": 0.0, "p99_agent": 0.0, "mean_agent": 0.0}
+    ij = np.floor(p / cell).astype(np.int64)
+    ij -= ij.min(axis=0)
+    key = ij[:, 0] * (int(ij[:, 1].max()) + 1) + ij[:, 1]
+    _, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    per_agent = cnt[inv] / (cell * cell)
+    return {"peak": float(cnt.max() / (cell * cell)), "p99_agent": float(np.quantile(per_agent, 0.99)),
+            "mean_agent": round(float(per_agent.mean()), 3),
+            "frac_agents_ge3": round(float((per_agent >= 3.0).mean()), 4)}
+
+
+def mean_path_points(sim):
+    """Mean path length in points over the active agents (the P of the
+    per-agent-update byte count 58 + 8 P)."""
+    off, cnt, _ = sim.paths()
+    act = sim.state()["active"].astype(bool)
+    return float(cnt[act].mean()) if act.any() else 0.0
 
 
 def copy_calibration(mb: int) -> None:
@@ -93,6 +120,15 @@ def main():
     print(f"[bench] spawned {n_spawned}", file=sys.stderr, flush=True)
     if not args.no_graph:
         sim.set_graph(True)
+    prerun = args.prerun if args.prerun is not None else int(getattr(wl, "info", {}).get("prerun_steps", 0))
+    if prerun:
+        t0 = time.time()
+        for i in range(prerun):
+            sim.Update()
+            if (i + 1) % 50 == 0:
+                sim.sync()
+                print(f"[bench] prerun step {i + 1}/{prerun} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        sim.sync()
 
     for _ in range(args.warmup):
         sim.Update()
@@ -118,6 +154,10 @@ def main():
     sim.sync()
     t1 = time.perf_counter()
     st1 = sim.stats()
+    # the crowd at the end of the timed window: local density and path length
+    window_state = sim.state()
+    density = local_density(window_state["position"], window_state["active"])
+    mean_path = mean_path_points(sim)
     # per-kernel HIP-event timing on the handle's stream, separate steps
     sim.set_timing(True)
     kt = np.zeros(4)
@@ -134,21 +174,29 @@ def main():
     names = ["grid_build", "attraction_knn", "orca_lp_integrate"]
     dom = int(np.argmax(kt_mean))
     n_per_step = updates / args.steps
-    # algorithmic HBM bytes per agent-update of each kernel: the state it
-    # must read and write once per agent (neighbour gathers served from
-    # cache and the grid's own scratch traffic are not counted)
-    bytes_per_agent = {
+    # algorithmic HBM bytes per agent-update: the state one update must read
+    # and write once (pos 8, vel 8, radius 4, preferred speed 4, path offset
+    # and count 8, active 1; the P path points the IRM scans, 8 P; writes of
+    # pos 8, vel 8, attraction 8, active 1), 58 + 8 P.  Neighbour gathers,
+    # the grid passes and verification outputs are not counted.
+    b_agent = 58.0 + 8.0 * mean_path
+    achieved = b_agent * n_per_step / (kt_mean[dom] * 1e-3) / 1e9
+    # each kernel's own minimum state traffic per agent, for comparison
+    own_bytes = {
         "grid_build": 1 + 8 + 8 + 4 + 4 + 4 + 16 + 8 + 4,
         "attraction_knn": 16 + 8 + 8 + 8 * mean_path + 8 + 4 + 8 + 8 + 4 * wl.params.k + 4,
         "orca_lp_integrate": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
     }
-    achieved = bytes_per_agent[names[dom]] * n_per_step / (kt_mean[dom] * 1e-3) / 1e9
     traffic, tsrc = _profiled(name, names[dom], "traffic")
     valu, _ = _profiled(name, names[dom], "valu_busy")
+    step_gbs = b_agent * n_per_step / (ms_per_step * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None if traffic is None else int(traffic),
             "traffic_source": tsrc, "valu_busy": valu,
-            "kernel": names[dom], "algorithmic_bytes_per_launch": int(bytes_per_agent[names[dom]] * n_per_step),
+            "kernel": names[dom], "bytes_per_agent_update": round(b_agent, 2),
+            "algorithmic_bytes_per_launch": int(b_agent * n_per_step),
+            "whole_step": {"achieved": round(step_gbs, 3), "frac": round(step_gbs / HBM_PEAK_GBS, 6)},
+            "kernel_own_bytes_per_agent": {k: round(v, 2) for k, v in own_bytes.items()},
             "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
 
     graph_on, graph_why = sim.graph_status()
@@ -181,7 +229,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": name, **wl.describe(), "agents_spawned": n_spawned,
-                   "mean_path_points": round(mean_path, 2)},
+                   "mean_path_points": round(mean_path, 2),
+                   "timed_window_steps": [prerun + args.warmup + 2, prerun + args.warmup + 1 + args.steps]},
+        "crowd": {"density_per_m2_1m_grid": density,
+                  "lp3d_rate": round((st1["lp3d"] - st0["lp3d"]) / max(1, updates), 5),
+                  "active_agents": int(window_state["active"].astype(bool).sum())},
         "launch": {"step_graphs": graph_on, "graph_fallback_reason": graph_why or None, "autotune": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,

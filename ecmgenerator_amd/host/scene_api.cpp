@@ -199,6 +199,105 @@ void add_rect(Scene& sc, const RectD& r) {
     sc.add_obstacle(xy, 4);
 }
 
+// Dense bottleneck: two opposing flows start in bands against the barrier
+// (x in [-2 - depth, -2] and [2, 2 + depth]) at kBandDensity agents/m^2 and
+// walk to goals in the band across (22 to 22 + depth m on the other side, y
+// within 100 m of the start), so every one of the 64 gaps of 8 m carries a
+// counterflow and the crowd in front of the barrier compresses within the
+// first seconds of the run.  Agents alternate between the flows.  A start
+// keeps the reference rule (no earlier agent within r, Simulator.cpp:295-311)
+// and, stricter, overlaps no earlier agent (distance >= r_i + r_j), so the
+// run starts without collisions; start and goal are at least 2r from every
+// obstacle.  Candidates are drawn one at a time against a 1 m grid of the
+// accepted starts; paths are planned afterwards on up to 16 threads and an
+// agent without a path is replaced in the next round.
+constexpr double kBandDensity = 2.0;   // 1M agents: bands 125 m deep
+
+int spawn_bottleneck_flows(const Scene& es, const Cfg& cf, std::mt19937_64& rng, int n_agents, ecm_scenario& sc) {
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<double> Nspd(1.3, 0.2);
+    const double H = cf.half;
+    const double kBand = std::min(H - 30.0, std::max(10.0, n_agents / (2.0 * 2.0 * H * kBandDensity)));
+    const double acell = 1.0;
+    const int aside = (int)std::ceil(2 * H / acell) + 1;
+    std::vector<std::vector<int>> agrid((size_t)aside * aside);
+    auto aidx = [&](double v) { return std::min(aside - 1, std::max(0, (int)std::floor((v + H) / acell))); };
+    std::vector<float> st, go, rr, sp;
+    std::vector<char> dead;
+    auto free_start = [&](double x, double y, double r) {
+        int gx = aidx(x), gy = aidx(y);
+        for (int yy = std::max(0, gy - 1); yy <= std::min(aside - 1, gy + 1); ++yy)
+            for (int xx = std::max(0, gx - 1); xx <= std::min(aside - 1, gx + 1); ++xx)
+                for (int j : agrid[(size_t)yy * aside + xx]) {
+                    if (dead[j]) continue;
+                    double dx = x - st[2 * j], dy = y - st[2 * j + 1], rs = r + rr[j];
+                    if (dx * dx + dy * dy < rs * rs) return false;
+                }
+        return true;
+    };
+    int have = 0;
+    for (int round = 0; round < 20 && have < n_agents; ++round) {
+        const size_t first = rr.size();
+        for (int i = 0; i < n_agents - have; ++i) {
+            const int side = (int)((first + i) & 1);      // 0: left flow walking +x, 1: right flow walking -x
+            const double sgn = side == 0 ? -1.0 : 1.0;
+            double r = 0.2 + 0.1 * U(rng);
+            double s = std::min(1.8, std::max(0.8, Nspd(rng)));
+            double sx = 0, sy = 0, gx = 0, gy = 0;
+            bool ok = false;
+            for (int t = 0; t < 20000 && !ok; ++t) {
+                sx = sgn * (2.0 + kBand * U(rng));
+                sy = -H + 2 * H * U(rng);
+                ok = es.clearance(sx, sy) >= 2 * r && free_start(sx, sy, r);
+            }
+            if (!ok) return sfail(ECM_E_STATE, "bottleneck band full after %zu agents", rr.size());
+            ok = false;
+            for (int t = 0; t < 1000 && !ok; ++t) {
+                gx = -sgn * (22.0 + kBand * U(rng));
+                gy = std::min(H - 2.0, std::max(-H + 2.0, sy + 200.0 * (U(rng) - 0.5)));
+                ok = es.clearance(gx, gy) >= 2 * r;
+            }
+            if (!ok) return sfail(ECM_E_STATE, "no free goal for agent %zu", rr.size());
+            agrid[(size_t)aidx(sy) * aside + aidx(sx)].push_back((int)rr.size());
+            st.push_back((float)sx); st.push_back((float)sy);
+            go.push_back((float)gx); go.push_back((float)gy);
+            rr.push_back((float)r);
+            sp.push_back((float)s);
+            dead.push_back(0);
+        }
+        const int m = (int)(rr.size() - first);
+        std::vector<std::vector<float>> paths(m);
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&]() {
+                for (int i; (i = next++) < m;) {
+                    size_t a = first + i;
+                    if (!es.plan(st[2 * a], st[2 * a + 1], go[2 * a], go[2 * a + 1], rr[a], paths[i])) paths[i].clear();
+                }
+            });
+        for (auto& t : th) t.join();
+        for (int i = 0; i < m; ++i) {
+            size_t a = first + i;
+            if (paths[i].size() < 4) {
+                dead[a] = 1;
+                continue;
+            }
+            sc.start.push_back(st[2 * a]); sc.start.push_back(st[2 * a + 1]);
+            sc.goal.push_back(go[2 * a]); sc.goal.push_back(go[2 * a + 1]);
+            sc.radius.push_back(rr[a]);
+            sc.speed.push_back(sp[a]);
+            sc.off.push_back((int)(sc.path.size() / 2));
+            sc.cnt.push_back((int)paths[i].size() / 2);
+            sc.path.insert(sc.path.end(), paths[i].begin(), paths[i].end());
+            ++have;
+        }
+    }
+    if (have < n_agents) return sfail(ECM_E_STATE, "could only place %d of %d agents", have, n_agents);
+    return 0;
+}
+
 }  // namespace
 
 static int generate(Cfg cf, uint64_t seed, int n_agents, ecm_scene** scene_out, ecm_scenario** out);
@@ -272,6 +371,18 @@ static int generate(Cfg cf, uint64_t seed, int n_agents, ecm_scene** scene_out, 
         return sfail(ECM_E_STATE, "ECM build failed: %s", m.c_str());
     }
     es->sc.ensure_rgrid();
+    if (cf.barrier) {
+        auto* sc = new ecm_scenario();
+        int rc = spawn_bottleneck_flows(es->sc, cf, rng, n_agents, *sc);
+        if (rc) {
+            delete sc;
+            delete es;
+            return rc;
+        }
+        *scene_out = es;
+        *out = sc;
+        return 0;
+    }
     // agents
     auto* sc = new ecm_scenario();
     std::normal_distribution<double> Nspd(1.3, 0.2);
@@ -289,22 +400,15 @@ static int generate(Cfg cf, uint64_t seed, int n_agents, ecm_scene** scene_out, 
             double r = 0.2 + 0.1 * U(rng);
             double s = std::min(1.8, std::max(0.8, Nspd(rng)));
             double sx, sy, gx, gy;
-            int side = cf.barrier ? ((have + i) & 1) : 0;
-            auto draw = [&](bool left_half, bool right_half, double& x, double& y) {
+            auto draw = [&](double& x, double& y) {
                 for (int t = 0; t < 1000; ++t) {
-                    double lo = -H, hi = H;
-                    if (left_half) hi = -10.0;
-                    if (right_half) lo = 10.0;
-                    x = lo + (hi - lo) * U(rng);
+                    x = -H + 2 * H * U(rng);
                     y = -H + 2 * H * U(rng);
                     if (es->sc.clearance(x, y) >= 2 * r) return true;
                 }
                 return false;
             };
-            bool ok;
-            if (cf.barrier) ok = draw(side == 0, side == 1, sx, sy) && draw(side == 1, side == 0, gx, gy);
-            else ok = draw(false, false, sx, sy) && draw(false, false, gx, gy);
-            if (!ok) continue;
+            if (!(draw(sx, sy) && draw(gx, gy))) continue;
             st.push_back((float)sx); st.push_back((float)sy);
             go.push_back((float)gx); go.push_back((float)gy);
             rr.push_back((float)r);

@@ -122,7 +122,8 @@ def _batch(name, seed, n_env=256):
 def build(name: str, seed: int = 1) -> Workload:
     """c0: 200 m, 40 obstacles, 5k agents; c1: 1000 m, 400 obstacles, 100k
     agents; c2: 2000 m, 1,500 obstacles (barrier with 64 gaps of 8 m), two
-    opposing flows of 500k agents; c0-ref: c0 with the reference's literal
+    opposing flows of 500k agents starting at 2 agents/m^2 in 125 m bands
+    against the barrier, goals in the band across; c0-ref: c0 with the reference's literal
     parameters (k = 5, T = 10 s, maxSpeed = preferred speed); c4: 256
     independent environments stepped together (c4-small: 8 of them)."""
     if name == "c0":
@@ -132,7 +133,11 @@ def build(name: str, seed: int = 1) -> Workload:
     if name == "c1":
         return _scenario(name, 1, seed)
     if name == "c2":
-        return _scenario(name, 2, seed)
+        wl = _scenario(name, 2, seed)
+        # the flows meet in the gaps within seconds; 150 steps (15 s) puts a
+        # timed window in the middle of the 300-step run, inside the jam
+        wl.info["prerun_steps"] = 150
+        return wl
     if name == "c4":
         return _batch(name, seed)
     if name == "c4-small":

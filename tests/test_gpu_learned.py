@@ -37,8 +37,10 @@ def test_mfma_forward_matches_oracle():
     yo, _ = M.forward(p, x, bf16=True)
     e = rel(y, yo)
     print(f"forward rel err {e:.2e} max abs {np.abs(y - yo).max():.2e}", flush=True)
-    assert e < 1e-2
-    assert np.abs(y - yo).max() < 0.05 * np.abs(yo).max()
+    # observed ~8e-8: the kernel and the oracle round the same BF16
+    # activations and differ only in FP32 summation order
+    assert e < 1e-6
+    assert np.abs(y - yo).max() < 1e-5 * np.abs(yo).max()
     net.close()
 
 
@@ -49,13 +51,13 @@ def test_loss_and_gradient_match_oracle():
     loss, g = net.loss_grad(x, lab, v, pos_weight=2.0, vel_weight=0.7)
     lo, go = M.loss_grad(p, x, lab, v, pos_weight=2.0, vel_weight=0.7, bf16=True)
     print(f"loss {loss:.6f} oracle {lo:.6f}", flush=True)
-    assert abs(loss - lo) <= 1e-3 * abs(lo)
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
     o = 0
     for name, shp in M.SHAPES:
         n = int(np.prod(shp))
         e = rel(g[o:o + n], go[o:o + n])
         print(f"grad {name} rel err {e:.2e}", flush=True)
-        assert e < 2e-2, name
+        assert e < 6e-4, name          # observed <= 6e-5
         o += n
     net.close()
 

@@ -71,15 +71,19 @@ def test_parity_pillar_forest_obstacle_overflow():
     sim = Simulator(ecm, obs, box, p)
     assert (sim.SpawnAgents(agents) >= 0).sum() > 300
     worst = {}
+    used_3d = 0
     for it in range(6):
         before = oracle_state(sim)
         ora = O.step(p, before, obs, ecm)
         sim.Update()
         res = compare_step(sim, before, ora)
+        used_3d += int(ora["used_3d"][before["active"].astype(bool)].sum())
         print(f"pillars step={it} {res} {sim.stats()}", flush=True)
         for k, v in res.items():
             worst[k] = max(worst.get(k, 0), v)
     st = sim.stats()
     sim.close()
     assert st["obst_overflow"] > 0, st
+    # the 3D LP ran on the device (k_lp3d) for the agents the oracle sent there
+    assert st["lp3d"] > 0 and st["lp3d"] == used_3d, (st, used_3d)
     assert all(v == 0 for v in worst.values()), worst
