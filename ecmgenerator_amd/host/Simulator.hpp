@@ -44,6 +44,11 @@ struct PositionComponent { float x; float y; };
 struct VelocityComponent { float dx; float dy; };
 struct ClearanceComponent { float clearance; };
 struct SpeedComponent { float speed; };
+// Simulator.h:51-57.  x and y point into the facade's host mirror of the
+// path pool; they stay valid until the next call that changes agents or
+// steps the simulation.  currentIndex is 0, as the reference never
+// advances it (Simulator.cpp:104,118).
+struct PathComponent { int currentIndex; int numPoints; float* x; float* y; };
 
 // Area.h:11-103
 enum SimAreaType { NONE, WALKABLE, SPAWN, GOAL, OBSTACLE };
@@ -101,7 +106,7 @@ class Simulator {
         const size_t N = (size_t)m_MaxNumEntities;
         m_Positions.resize(N); m_Velocities.resize(N); m_PreferredVelocities.resize(N);
         m_AttractionPoints.resize(N); m_Clearances.resize(N); m_Active.resize(N);
-        m_Dirty = true;
+        m_Dirty = m_PathsDirty = true;
     }
 
     // Simulator::SpawnAgent (Simulator.cpp:168-200): plans the path with the
@@ -109,16 +114,16 @@ class Simulator {
     // location invalid or no path exists.
     int SpawnAgent(float sx, float sy, float gx, float gy, float clearance, float preferredSpeed) {
         const int id = SpawnOne(sx, sy, gx, gy, clearance, preferredSpeed, 1);
-        m_Dirty = true;
+        m_Dirty = m_PathsDirty = true;
         return id;
     }
     // DestroyAgent (Simulator.cpp:202-208)
-    void DestroyAgent(int idx) { check(ecm_sim_destroy_agent(m_Sim, idx), "destroy"); m_Dirty = true; }
+    void DestroyAgent(int idx) { check(ecm_sim_destroy_agent(m_Sim, idx), "destroy"); m_Dirty = m_PathsDirty = true; }
     // Update (Simulator.cpp:314-323): spawn areas first, as the reference
     void Update(float /*dt*/) {
         UpdateSpawnAreas();
         check(ecm_sim_step(m_Sim, 1), "step");
-        m_Dirty = true;
+        m_Dirty = m_PathsDirty = true;
     }
 
     // ValidSpawnLocation (Simulator.cpp:295-311): no active agent closer than
@@ -252,10 +257,10 @@ class Simulator {
                 area.timeSinceLastSpawn[ga] -= (float)agentsToSpawn / rate;
             }
         }
-        if (spawned) m_Dirty = true;
+        if (spawned) m_Dirty = m_PathsDirty = true;
     }
     // Reset (Simulator.cpp:325-333)
-    void Reset() { check(ecm_sim_reset(m_Sim), "reset"); m_Dirty = true; }
+    void Reset() { check(ecm_sim_reset(m_Sim), "reset"); m_Dirty = m_PathsDirty = true; }
 
     // getters (Simulator.h:114-126)
     int GetNumAgents() const { int n = 0; check(ecm_sim_num_agents(m_Sim, &n), "num_agents"); return n; }
@@ -266,6 +271,35 @@ class Simulator {
     PositionComponent* GetAttractionPointData() { sync(); return m_AttractionPoints.data(); }
     ClearanceComponent* GetClearanceData() { sync(); return m_Clearances.data(); }
     bool* GetActiveFlags() { sync(); return reinterpret_cast<bool*>(m_Active.data()); }
+    // GetPathData (Simulator.h:119): one PathComponent per slot, read back
+    // with ecm_sim_read_paths when a spawn, replan or removal may have
+    // changed a path since the last read.
+    PathComponent* GetPathData() {
+        if (m_PathsDirty && m_Sim) {
+            const size_t N = (size_t)m_MaxNumEntities;
+            std::vector<int> off(N), cnt(N);
+            size_t used = 0;
+            check(ecm_sim_read_paths(m_Sim, off.data(), cnt.data(), nullptr, 0, &used), "read_paths");
+            std::vector<float> pool(2 * used + 2);
+            check(ecm_sim_read_paths(m_Sim, off.data(), cnt.data(), pool.data(), used, &used), "read_paths");
+            m_PathX.resize(used + 1);
+            m_PathY.resize(used + 1);
+            for (size_t j = 0; j < used; ++j) {
+                m_PathX[j] = pool[2 * j];
+                m_PathY[j] = pool[2 * j + 1];
+            }
+            m_Paths.resize(N);
+            for (size_t i = 0; i < N; ++i) {
+                const bool ok = cnt[i] > 0 && off[i] >= 0 && (size_t)off[i] + (size_t)cnt[i] <= used;
+                m_Paths[i].currentIndex = 0;
+                m_Paths[i].numPoints = ok ? cnt[i] : 0;
+                m_Paths[i].x = ok ? m_PathX.data() + off[i] : nullptr;
+                m_Paths[i].y = ok ? m_PathY.data() + off[i] : nullptr;
+            }
+            m_PathsDirty = false;
+        }
+        return m_Paths.data();
+    }
     float GetSimulationStepTime() const { return m_Params.dt; }
     ecm_sim* Handle() const { return m_Sim; }
 
@@ -349,6 +383,9 @@ class Simulator {
     ecm_sim_params m_Params{};
     ecm_sim* m_Sim = nullptr;
     bool m_Dirty = true;
+    bool m_PathsDirty = true;
+    std::vector<PathComponent> m_Paths;
+    std::vector<float> m_PathX, m_PathY;
     std::vector<PositionComponent> m_Positions, m_AttractionPoints;
     std::vector<VelocityComponent> m_Velocities, m_PreferredVelocities;
     std::vector<ClearanceComponent> m_Clearances;

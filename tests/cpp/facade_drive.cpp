@@ -20,9 +20,39 @@ int main() {
     ecm_amd::Simulation::Simulator sim(scene, -30, -30, 30, 30, 256, 0.1f);
     sim.Initialize();
     int spawned = 0;
-    for (int i = 0; i < n; ++i) spawned += sim.SpawnAgent(st[2 * i], st[2 * i + 1], go[2 * i], go[2 * i + 1], r[i], s[i]) >= 0;
+    std::vector<int> agent_of(256, -1);
+    for (int i = 0; i < n; ++i) {
+        const int id = sim.SpawnAgent(st[2 * i], st[2 * i + 1], go[2 * i], go[2 * i + 1], r[i], s[i]);
+        if (id >= 0) { agent_of[id] = i; ++spawned; }
+    }
     std::printf("spawned %d of %d, GetNumAgents %d, GetLastIndex %d\n", spawned, n, sim.GetNumAgents(), sim.GetLastIndex());
     if (spawned < n / 2 || sim.GetNumAgents() != spawned) return 3;
+    // walk the paths as ECMRenderer::DrawPaths does (ECMRenderer.cpp:801-830):
+    // every active slot up to GetLastIndex, segments j -> j+1; each path runs
+    // from the agent's start to its goal
+    auto walk_paths = [&](bool from_start) {
+        const int lastIdx = sim.GetLastIndex();
+        const bool* activeFlags = sim.GetActiveFlags();
+        const auto* paths = sim.GetPathData();
+        int walked = 0, segments = 0, bad = 0;
+        for (int i = 0; i <= lastIdx; i++) {
+            if (!activeFlags[i]) continue;
+            const auto path = paths[i];
+            const int a = agent_of[i];
+            if (path.numPoints < 1 || path.currentIndex != 0 || a < 0) { ++bad; continue; }
+            for (int j = 0; j < path.numPoints - 1; j++) {
+                const float len = std::hypot(path.x[j + 1] - path.x[j], path.y[j + 1] - path.y[j]);
+                if (!std::isfinite(len)) ++bad;
+                ++segments;
+            }
+            if (path.x[path.numPoints - 1] != go[2 * a] || path.y[path.numPoints - 1] != go[2 * a + 1]) ++bad;
+            if (from_start && (path.x[0] != st[2 * a] || path.y[0] != st[2 * a + 1])) ++bad;
+            ++walked;
+        }
+        std::printf("paths: %d walked, %d segments, %d bad\n", walked, segments, bad);
+        return bad == 0 && walked > 0 && segments >= walked;
+    };
+    if (!walk_paths(true)) return 6;
     std::vector<float> x0(2 * 256);
     auto* p0 = sim.GetPositionData();
     for (int i = 0; i < 256; ++i) { x0[2 * i] = p0[i].x; x0[2 * i + 1] = p0[i].y; }
@@ -40,6 +70,8 @@ int main() {
     std::printf("after 50 steps: alive %d, mean displacement %.3f m, non-finite %d\n", alive, moved / spawned, bad);
     ecm_scenario_destroy(sc);
     if (bad || moved / spawned < 1.0) return 4;
+    // replanned paths start where the agent was; every path still ends at its goal
+    if (!walk_paths(false)) return 7;
     sim.Reset();
     if (sim.GetNumAgents() != 0) return 5;
     std::printf("facade ok\n");
