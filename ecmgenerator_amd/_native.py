@@ -123,6 +123,7 @@ SIGNATURES = {
     "ecm_sim_log_step": (_I, [_P, _P, _PI]),
     "ecm_mlp_train": (_I, [_P, _I, _I, _F, _F, _F, C.c_uint64, _PF]),
     "ecm_mlp_forward": (_I, [_P, _PF, _I, _PF]),
+    "ecm_mlp_forward_noalloc": (_I, [_P, _P, _I, _P]),
     "ecm_mlp_loss_grad": (_I, [_P, _PF, C.POINTER(C.c_uint32), _PF, _I, _F, _F, _PF, _PF]),
     "ecm_sim_set_learned": (_I, [_P, _P, _F]),
 }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -278,6 +279,51 @@ extern "C" int ecm_mlp_forward(ecm_mlp* m, const float* x, int n, float* y) {
     HIPCHK(launch_mlp_infer(m->xb.p, 0, nullptr, n, m->frags.p, m->pbias.p, 0.0f, m->lb.p, nullptr, m->y.p, m->n_cu,
                             m->stream));
     HIPCHK(hipMemcpyAsync(y, m->y.p, (size_t)n * kOut * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+// ------------------------------------------- forward without allocation
+// Scratch of ecm_mlp_forward_noalloc: the BF16 features ([n][64]) and the
+// proposal masks ([n]) of up to ECM_MLP_NOALLOC_MAX_SAMPLES samples.  Every
+// GPU holds its own copy until the process ends.  Nothing in it has to
+// start at zero: the conversion kernel writes every feature the network
+// reads and the network writes every mask.
+constexpr size_t kNoAllocFeatBytes = (size_t)ECM_MLP_NOALLOC_MAX_SAMPLES * kInPad * 2;
+constexpr size_t kNoAllocBytes = kNoAllocFeatBytes + (size_t)ECM_MLP_NOALLOC_MAX_SAMPLES * 4;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on device `dev`, looked up once per device
+// with that device current (each GPU holds the array at its own address).
+static int scratch_address(int dev, unsigned char** out) {
+    static std::mutex mu;
+    static std::vector<unsigned char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev < 0) return fail(ECM_E_ARG, "bad device %d", dev);
+    if ((size_t)dev >= addr.size()) addr.resize((size_t)dev + 1, nullptr);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ECM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        addr[dev] = (unsigned char*)p;
+    }
+    *out = addr[dev];
+    return 0;
+}
+
+extern "C" int ecm_mlp_forward_noalloc(ecm_mlp* m, const float* x, int n, float* y) {
+    if (!m || !x || !y || n < 1) return fail(ECM_E_ARG, "bad arguments");
+    if (n > ECM_MLP_NOALLOC_MAX_SAMPLES)
+        return fail(ECM_E_ARG, "%d samples exceed the %d of the static scratch", n, ECM_MLP_NOALLOC_MAX_SAMPLES);
+    HIPCHK(hipSetDevice(m->dev));
+    unsigned char* base = nullptr;
+    int rc = scratch_address(m->dev, &base);
+    if (rc) return rc;
+    uint16_t* xb = reinterpret_cast<uint16_t*>(base);
+    uint32_t* mask = reinterpret_cast<uint32_t*>(base + kNoAllocFeatBytes);
+    HIPCHK(launch_f32_to_bf16_rne(x, xb, (long)n * kInPad, m->stream));
+    HIPCHK(launch_mlp_infer(xb, 0, nullptr, n, m->frags.p, m->pbias.p, 0.0f, mask, nullptr, y, m->n_cu, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     return 0;
 }

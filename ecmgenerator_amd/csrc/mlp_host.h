@@ -28,6 +28,7 @@ hipError_t launch_colsum(const float* X, int B, int N, float* out, hipStream_t s
 hipError_t launch_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                        float bc1, float bc2, __bf16* wb, hipStream_t st);
 hipError_t launch_to_bf16(const float* w, __bf16* wb, int n, hipStream_t st);
+hipError_t launch_f32_to_bf16_rne(const float* x, uint16_t* out, long n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, long n, hipStream_t st);
 hipError_t launch_gather(const int64_t* perm, int B, const void* X, const uint32_t* L, const float2* V, void* Xb,
                          uint32_t* Lb, float2* Vb, hipStream_t st);

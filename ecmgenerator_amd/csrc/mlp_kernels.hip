@@ -283,6 +283,16 @@ __global__ void k_to_bf16(const float* w, __bf16* wb, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) wb[i] = (__bf16)w[i];
 }
 
+// FP32 -> BF16, round to nearest even, NaN -> quiet NaN: the same bit
+// arithmetic as the host staging of ecm_mlp_forward, so both paths feed
+// the network identical operands.
+__global__ void k_f32_to_bf16_rne(const float* x, uint16_t* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t u = __float_as_uint(x[i]);
+        out[i] = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)0x7fc0 : (uint16_t)((u + (((u >> 16) & 1u) + 0x7fffu)) >> 16);
+    }
+}
+
 __global__ void k_fill_f32(float* p, float v, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -341,6 +351,11 @@ hipError_t launch_mlp_infer(const void* feat, int tiled, const uint32_t* n_dev, 
                             hipStream_t st) {
     InferArgs a{(const bf16x8*)feat, tiled, n_dev, n_host, (const bf16x8*)frags, bias, theta, mask, vel, raw};
     hipLaunchKernelGGL(k_mlp_infer, dim3(n_cu), dim3(1024), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_f32_to_bf16_rne(const float* x, uint16_t* out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_f32_to_bf16_rne, dim3((unsigned)std::min<long>(cdiv(n, 256), 4096)), dim3(256), 0, st, x, out, n);
     return hipGetLastError();
 }
 

@@ -116,6 +116,13 @@ class ProposalNet:
         N.check(self._L.ecm_mlp_forward(self._h, N.fptr(x), x.shape[0], N.fptr(y)))
         return y
 
+    def forward_device(self, x_ptr: int, n: int, y_ptr: int) -> None:
+        """Forward pass on device buffers without any allocation
+        (ecm_mlp_forward_noalloc): x_ptr holds n*64 FP32 features, y_ptr
+        receives n*16 raw outputs, both device addresses on this network's
+        GPU."""
+        N.check(self._L.ecm_mlp_forward_noalloc(self._h, C.c_void_p(x_ptr), int(n), C.c_void_p(y_ptr)))
+
     def loss_grad(self, x, label, v, pos_weight=2.0, vel_weight=1.0):
         x = np.ascontiguousarray(x, np.float32)
         label = np.ascontiguousarray(label, np.uint32)

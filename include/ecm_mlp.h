@@ -74,6 +74,18 @@ int ecm_mlp_train(ecm_mlp* m, int epochs, int batch, float lr, float pos_weight,
  * device) with the inference kernel: y (n*16) raw outputs. */
 int ecm_mlp_forward(ecm_mlp* m, const float* x, int n, float* y);
 
+/* ecm_mlp_forward for device buffers, allocating nothing: x (n*64 FP32)
+ * and y (n*16 raw outputs) are device memory of the handle's GPU, and the
+ * BF16 features and proposal masks go to a static scratch array of the
+ * library (one per GPU, ECM_MLP_NOALLOC_MAX_SAMPLES samples = 4.125 MiB,
+ * held until the process ends).  No hipMalloc, hipFree or other allocating
+ * call; n above the maximum is rejected with ECM_E_ARG.  Calls that
+ * overlap on one GPU share the scratch and race.  Returns after y is
+ * written.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops). */
+#define ECM_MLP_NOALLOC_MAX_SAMPLES 32768
+int ecm_mlp_forward_noalloc(ecm_mlp* m, const float* x, int n, float* y);
+
 /* Loss and gradient (n_params FP32, same order as the parameters) of n
  * host samples, with the training kernels (one batch, no update). */
 int ecm_mlp_loss_grad(ecm_mlp* m, const float* x, const uint32_t* label, const float* v, int n, float pos_weight,
