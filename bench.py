@@ -323,7 +323,7 @@ def learned_leg(sim, wl, snap, wpath, args):
     return {"value": round(upd / (t1 - t0), 1), "unit": "agent-updates/s",
             "ms_per_step": round(1000.0 * (t1 - t0) / args.steps, 4),
             "acceptance": round(acc / max(tried, 1), 4), "tried": tried, "accepted": acc,
-            "fallback_queue": st1["lp3d"] - st0["lp3d"],
+            "rejected_full_2d_lp": tried - acc, "lp3d": st1["lp3d"] - st0["lp3d"],
             "max_dv_vs_plain": max_dv, "max_dp_vs_plain": max_dp, "tolerance_dv": 1e-4,
             "theta": theta, "weights": os.path.relpath(wpath, ROOT),
             "kernel_ms": {"grid_build": round(float(kt[0]), 4), "attraction_knn": round(float(kt[1]), 4),

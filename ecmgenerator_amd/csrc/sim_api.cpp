@@ -28,7 +28,7 @@ hipError_t launch_unsort(const DevState& s, hipStream_t st);
 hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
                             const unsigned long long* acc, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
@@ -183,6 +183,7 @@ struct ecm_sim {
     bool envs_dirty = true;
     uint32_t n_keys = 0;
     int maxo = 16;                 // longest obstacle bucket list of any environment (16 / 32 / 64)
+    int n_cu = 256;                // compute units of the device (persistent queue grids)
     float max_radius = 0.0f, max_pref_speed = 0.0f;   // over every agent spawned
     DBuf<uint16_t> agent_env;      // environment of each slot (with more than one environment)
     std::vector<uint16_t> h_env;
@@ -194,6 +195,7 @@ struct ecm_sim {
     DBuf<int4> q3d_info;
     DBuf<float2> q3d_vel;
     DBuf<float4> q3d_con;
+    DBuf<int> q3d_big;
     int q3d_stride = 0;
 
     // host mirrors of the slot pool
@@ -283,7 +285,7 @@ static void free_all(ecm_sim* s) {
     s->fwd.release(); s->prev_of.release(); s->pos_of.release();
     s->new_ids.release(); s->dead_ids.release();
     s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
-    s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release();
+    s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release(); s->q3d_big.release();
     for (auto& e : s->envs) e->release();
     s->d_envs.release();
     s->agent_env.release();
@@ -366,6 +368,11 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (hipGetDevice(&s->dev) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(ECM_E_HIP, "stream creation failed");
+    }
+    {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, s->dev) == hipSuccess && pr.multiProcessorCount > 0)
+            s->n_cu = pr.multiProcessorCount;
     }
     if (!rc) rc |= s->pos.alloc(N);
     if (!rc) rc |= s->vel.alloc(N);
@@ -1203,6 +1210,7 @@ static DevState make_state(ecm_sim* s) {
     }
     d.q3d_agent = s->q3d_agent.p; d.q3d_info = s->q3d_info.p; d.q3d_vel = s->q3d_vel.p;
     d.q3d_con = s->q3d_con.p; d.con_stride = s->q3d_stride;
+    d.q3d_cap = (uint32_t)s->prm.max_agents; d.q3d_big = s->q3d_big.p;
     d.k = s->prm.k;
     d.time_horizon = s->prm.time_horizon;
     d.time_horizon_obst = s->prm.time_horizon_obst;
@@ -1346,6 +1354,7 @@ static int prepare_step(ecm_sim* s, DevState* out) {
             if (!rc) rc = s->q3d_info.alloc(N);
             if (!rc) rc = s->q3d_vel.alloc(N);
             if (!rc) rc = s->q3d_con.alloc(N * (size_t)stride);
+            if (!rc) rc = s->q3d_big.alloc(N);
             if (rc) return rc;
             s->q3d_stride = stride;
         }
@@ -1380,7 +1389,7 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
                                      s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu, s->stream));
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->stream));
+    HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->n_cu, s->stream));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
     return 0;
 }
@@ -1807,6 +1816,7 @@ extern "C" int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n) {
     st[2] = acc[2] + cur[2];        // 3D-LP queue
     st[3] = acc[4] + cur[4];        // locate fallbacks
     st[6] = acc[3] + cur[3];        // obstacle overflow queue
+    st[10] = acc[6] + cur[6];       // 3D-LP entries solved from the LDS tile
     st[7] = col[0];
     st[8] = col[1];
     st[9] = col[2];

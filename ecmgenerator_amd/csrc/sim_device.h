@@ -124,7 +124,9 @@ struct DevState {
     // (x, y); (id, 0, 0, 1) is a DestroyAgent.
     int4* events;         uint32_t* events_n;
     uint32_t* stats;      // per step: [0] 3D-LP queue length, [1] obstacle-overflow queue
-                          // length, [2] locate fallbacks to the exact-y list, [3..5] unused
+                          // length, [2] locate fallbacks to the exact-y list, [3] blocks of
+                          // k_attr_knn done, [4] 3D-LP entries with more obstacle constraints
+                          // than the register tile holds, [5] unused
     uint32_t* wstats;     // per step, [64][4] slots: learned proposals tried, accepted,
                           // constraints proposed, unused
     unsigned long long* acc;   // [16] totals of the finished steps; [12] steps started,
@@ -139,8 +141,11 @@ struct DevState {
     int* q3d_agent;         // sorted position
     int4* q3d_info;         // (n constraints, n obstacle constraints, failed index, 0)
     float2* q3d_vel;        // 2D LP result at the failure
-    float4* q3d_con;        // [queue][con_stride] constraints (nx, ny, px, py)
-    int con_stride;         // MAXO + k
+    float4* q3d_con;        // constraints (nx, ny, px, py), slot-major: constraint c of entry q
+                            // at c * q3d_cap + q (q3d_slot)
+    int con_stride;         // constraints per entry: MAXO + k
+    uint32_t q3d_cap;       // entries the queue holds (max_agents)
+    int* q3d_big;           // entries with more obstacle constraints than the register tile
 
     // ---- learned proposal ----
     uint4* feat;                // BF16 network inputs by sorted position t: chunk c (8 BF16) at

@@ -22,6 +22,8 @@
 
 #include <limits.h>
 
+#include <algorithm>
+
 namespace ecm {
 
 static constexpr uint32_t kInvalidKey = 0xffffffffu;
@@ -918,6 +920,12 @@ __device__ __forceinline__ Con con_ld(const float4* lds, const Tile& T, int slot
 __device__ __forceinline__ void con_st(float4* lds, const Tile& T, int slot, V2 p, V2 n) {
     lds[T.at(slot)] = make_float4(n.x, n.y, p.x, p.y);
 }
+// Constraint c of 3D-LP queue entry q (slot-major, so consecutive entries
+// held by consecutive lanes are consecutive float4s).
+__device__ __forceinline__ float4* q3d_slot(const DevState& s, uint32_t q, int c) {
+    return s.q3d_con + (size_t)c * s.q3d_cap + q;
+}
+
 // Constraint::Contains, methodB (ORCA.h:49-60)
 __device__ __forceinline__ bool con_contains(const Con& c, V2 q) { return det(right(c.n), c.p - q) <= 0.0f; }
 __device__ __forceinline__ float std_min(float a, float b) { return (b < a) ? b : a; }
@@ -1456,28 +1464,22 @@ __device__ __forceinline__ void orca_agent(const DevState& s, int t, float4* lds
         wave_count(ws + 0, true);
         wave_count(ws + 1, ok);
         wave_sum5(ws + 2, (uint32_t)__popc(cm));
-        if (!ok) {
-            uint32_t q = wave_alloc(&s.stats[0]);
-            s.q3d_agent[q] = t;
-            s.q3d_info[q] = make_int4(nc, nObst, -1, 0);
-            float4* dst = s.q3d_con + (size_t)q * s.con_stride;
-            for (int c = 0; c < nc; ++c) dst[c] = lds[TL.at(c)];
+        if (ok) {
+            integrate(s, t, position, velocity, o);
             return;
         }
-        integrate(s, t, position, velocity, o);
-        return;
+        // rejected: the full 2D LP, as the plain pass
     }
 
     int fail = lp2d(lds, TL, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
     if (fail < nc) {
-        // RandomizedLP3D runs in k_lp3d on a compacted queue: park the
-        // constraints, the 2D result and the failing index
+        // RandomizedLP3D runs on a compacted queue: park the constraints,
+        // the 2D result and the failing index
         uint32_t q = wave_alloc(&s.stats[0]);
         s.q3d_agent[q] = t;
         s.q3d_info[q] = make_int4(nc, nObst, fail, 0);
         s.q3d_vel[q] = make_float2(o.x, o.y);
-        float4* dst = s.q3d_con + (size_t)q * s.con_stride;
-        for (int c = 0; c < nc; ++c) dst[c] = lds[TL.at(c)];
+        for (int c = 0; c < nc; ++c) *q3d_slot(s, q, c) = lds[TL.at(c)];
         return;
     }
     if (MODE == kLog) {
@@ -1545,35 +1547,178 @@ __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
         orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, Tile{tid, INT_MAX, 0}, nullptr);
 }
 
-// RandomizedLP3D (ORCA.cpp:592-669) for the agents whose 2D LP failed
-// (after the full 2D LP for a rejected proposal), then integration.  Fixed
-// grid, strided loop over the queue.
+// RandomizedLP3D (ORCA.cpp:592-669) with the constraints in an LDS tile,
+// for the 3D-LP queue entries with more obstacle constraints than the
+// register tile of k_lp3d_reg holds (listed by it in q3d_big), then
+// integration.  Fixed grid, strided loop.
 template <int K, int MAXO>
 __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     __shared__ float4 lds[(MAXO + 2 * K) * 64];
-    const uint32_t nq = s.stats[0];
+    const uint32_t nq = s.stats[4];
     const int tid = threadIdx.x;
     const Tile T{tid, INT_MAX, 0};
     // entries go lane-major across the blocks, as in k_orca_ovf
-    for (uint32_t q = tid * gridDim.x + blockIdx.x; q < nq; q += gridDim.x * 64) {
+    for (uint32_t i = tid * gridDim.x + blockIdx.x; i < nq; i += gridDim.x * 64) {
+        const uint32_t q = (uint32_t)s.q3d_big[i];
         const int t = s.q3d_agent[q];
-        int4 info = s.q3d_info[q];
-        const float4* src = s.q3d_con + (size_t)q * s.con_stride;
-        for (int c = 0; c < info.x; ++c) lds[T.at(c)] = src[c];
+        const int4 info = s.q3d_info[q];
+        for (int c = 0; c < info.x; ++c) lds[T.at(c)] = *q3d_slot(s, q, c);
         const float4 pr = s.sorted_pr[t];
         const float4 sv = s.sorted_v[t];
-        const float2 v0 = make_float2(sv.x, sv.y);
         const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
-        V2 o;
-        if (info.z < 0) {
-            const float2 pv = s.pref_vel_s[t];
-            info.z = lp2d(lds, T, info.x, info.x, 0, mk(pv.x, pv.y), maxSpeed, false, o);
-        } else {
-            const float2 ov = s.q3d_vel[q];
-            o = mk(ov.x, ov.y);
+        const float2 ov = s.q3d_vel[q];
+        V2 o = mk(ov.x, ov.y);
+        lp3d(lds, T, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
+        integrate(s, t, mk(pr.x, pr.y), mk(sv.x, sv.y), o);
+    }
+}
+
+// ORCA::RandomizedLP with useDirOpt (ORCA.cpp:428-587), the form the 3D LP
+// calls, over registers: the list is OB obstacle constraints (the first
+// nObst valid) followed by K bisectors (valid where bit j of pval is set),
+// in that order, fully unrolled so every constraint stays in a register.
+// Returns false where the reference returns a failing index.
+template <int OB, int K>
+__device__ __forceinline__ bool lp_dir_reg(const V2 (&on)[OB], const V2 (&op)[OB], int nObst, const V2 (&pn)[K],
+                                           const V2 (&pp)[K], uint32_t pval, V2 opt, float maxSpeed, V2& out) {
+    out = opt * maxSpeed;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < OB + K; ++i) {
+        const bool vi = i < OB ? (i < nObst) : ((pval >> (i - OB)) & 1u) != 0u;
+        const V2 hn = i < OB ? on[i < OB ? i : 0] : pn[i < OB ? 0 : i - OB];
+        const V2 hp = i < OB ? op[i < OB ? i : 0] : pp[i < OB ? 0 : i - OB];
+        if (ok && vi && det(right(hn), hp - out) > 0.0f) {
+            const V2 dir = right(hn);
+            const float dpd = dot(dir, hp);
+            const float disc = dpd * dpd + maxSpeed * maxSpeed - dot(hp, hp);
+            if (disc <= 0.0f) {
+                ok = false;
+            } else {
+                const float sq = sqrtf(disc);
+                float lft = -dpd - sq;
+                float rgt = -dpd + sq;
+                bool bad = false;
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    const bool vj = j < OB ? (j < nObst) : ((pval >> (j - OB)) & 1u) != 0u;
+                    const V2 jn = j < OB ? on[j < OB ? j : 0] : pn[j < OB ? 0 : j - OB];
+                    const V2 jp = j < OB ? op[j < OB ? j : 0] : pp[j < OB ? 0 : j - OB];
+                    if (!bad && vj) {
+                        const float den = det(right(hn), right(jn));
+                        const float num = det(right(jn), hp - jp);
+                        if (fabsf(den) <= kEps) {
+                            if (num < 0.0f) bad = true;
+                        } else {
+                            const float t = num / den;
+                            if (den >= 0.0f) rgt = std_min(rgt, t);
+                            else lft = std_max(lft, t);
+                            if (lft > rgt) bad = true;
+                        }
+                    }
+                }
+                if (bad) ok = false;
+                else if (dot(opt, right(hn)) > 0) out = hp + right(hn) * rgt;
+                else out = hp + right(hn) * lft;
+            }
         }
-        if (info.z < info.x) lp3d(lds, T, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
-        integrate(s, t, mk(pr.x, pr.y), mk(v0.x, v0.y), o);
+    }
+    return ok;
+}
+
+// ORCA::RandomizedLP3D (ORCA.cpp:592-669) over registers: OB obstacle
+// constraints (nObst valid) and K agent constraints (na valid), failing
+// index given as a position in that list (vfail).  The loop over the
+// candidate constraint runs over list positions, the same for every lane of
+// the wavefront, so the bisector count and the inner LP's length are
+// wave-uniform; a lane takes part from its failing position on.
+template <int OB, int K>
+__device__ __forceinline__ void lp3d_reg(const DevState& s, uint32_t q, const V2 (&on)[OB], const V2 (&op)[OB],
+                                         const V2 (&an)[K], const V2 (&ap)[K], int nObst, int na, int vfail,
+                                         float maxSpeed, V2& out) {
+    float maxPen = 0.0f;
+#pragma unroll 1
+    for (int v = 0; v < OB + K; ++v) {
+        const int jv = v - OB;
+        const bool in_list = v < OB ? (v < nObst) : (jv < na);
+        if (!(v >= vfail && in_list)) continue;
+        // the candidate constraint ci, read again from the queue entry: a
+        // register select by the loop position would index the arrays
+        const float4 c4 = *q3d_slot(s, q, v < OB ? v : nObst + jv);
+        const V2 cn = mk(c4.x, c4.y), cp = mk(c4.z, c4.w);
+        const V2 dir = right(cn);
+        if (!(det(dir, cp - out) > maxPen)) continue;
+        // bisectors with the agent constraints before ci
+        V2 pn[K], pp[K];
+        uint32_t pval = 0u;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            pn[j] = mk(0.0f, 0.0f);
+            pp[j] = mk(0.0f, 0.0f);
+            if (j < jv) {
+                const float d = det(right(cn), right(an[j]));
+                if (fabsf(d) <= kEps) {
+                    if (!(dot(cn, an[j]) > 0)) {
+                        pp[j] = (cp + ap[j]) * 0.5f;
+                        pn[j] = normalized(an[j] - cn);
+                        pval |= 1u << j;
+                    }
+                } else {
+                    const float tt = det(right(an[j]), cp - ap[j]) / d;
+                    pp[j] = cp + right(cn) * tt;
+                    pn[j] = normalized(an[j] - cn);
+                    pval |= 1u << j;
+                }
+            }
+        }
+        const V2 temp = out;
+        if (!lp_dir_reg<OB, K>(on, op, nObst, pn, pp, pval, cn, maxSpeed, out)) out = temp;
+        maxPen = det(dir, cp - out);
+    }
+}
+
+// The 3D-LP queue (agents whose 2D LP failed), then integration: constraints
+// in registers (up to OB obstacle and K agent constraints; entries with more
+// obstacle constraints go to q3d_big for k_lp3d).  Each wavefront takes 64
+// consecutive entries (coalesced slot-major loads); wavefronts are dealt to
+// the blocks round robin so that a short queue still spreads over the chip.
+template <int K, int OB>
+__global__ __launch_bounds__(256) void k_lp3d_reg(DevState s) {
+    const uint32_t nq = s.stats[0];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t nw = gridDim.x * wpb;
+    for (uint32_t w = blockIdx.x + gridDim.x * (threadIdx.x >> 6); w * 64u < nq; w += nw) {
+        const uint32_t q = w * 64u + lane;
+        if (q >= nq) continue;
+        const int4 info = s.q3d_info[q];
+        const int nObst = info.y, na = info.x - info.y;
+        if (nObst > OB) {
+            s.q3d_big[wave_alloc(&s.stats[4])] = (int)q;
+            continue;
+        }
+        const int t = s.q3d_agent[q];
+        V2 on[OB], op[OB], an[K], ap[K];
+#pragma unroll
+        for (int c = 0; c < OB; ++c) {
+            const float4 v = c < nObst ? *q3d_slot(s, q, c) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            on[c] = mk(v.x, v.y);
+            op[c] = mk(v.z, v.w);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const float4 v = j < na ? *q3d_slot(s, q, nObst + j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            an[j] = mk(v.x, v.y);
+            ap[j] = mk(v.z, v.w);
+        }
+        const float4 pr = s.sorted_pr[t];
+        const float4 sv = s.sorted_v[t];
+        const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
+        const float2 ov = s.q3d_vel[q];
+        V2 o = mk(ov.x, ov.y);
+        const int vfail = info.z < nObst ? info.z : OB + (info.z - nObst);
+        lp3d_reg<OB, K>(s, q, on, op, an, ap, nObst, na, vfail, maxSpeed, o);
+        integrate(s, t, mk(pr.x, pr.y), mk(sv.x, sv.y), o);
     }
 }
 
@@ -1669,20 +1814,29 @@ static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStr
     else hipLaunchKernelGGL((k_orca<K, kPlain, OSLOTS>), g, b, 0, st, s);
 }
 
+// Obstacle constraints held in registers by k_lp3d_reg; its persistent
+// grid is kLp3dRegBlocks workgroups of 256 per CU.
+constexpr int kLp3dRegObst = 4;
+constexpr int kLp3dRegBlocks = 8;
+
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, hipStream_t st) {
+static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
     if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
     else if (oslots <= 2) launch_orca_main<K, 2>(s, mode, g, b, st);
     else launch_orca_main<K, 4>(s, mode, g, b, st);
+    const dim3 gr(std::max(1, n_cu) * kLp3dRegBlocks), br(256);
     if (maxo <= 16) {
         hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 16>), gq, b, 0, st, s);
     } else if (maxo <= 32) {
         hipLaunchKernelGGL((k_orca_ovf<K, 32>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 32>), gq, b, 0, st, s);
     } else {
         hipLaunchKernelGGL((k_orca_ovf<K, 64>), gq, b, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 64>), gq, b, 0, st, s);
     }
 }
@@ -1704,10 +1858,10 @@ hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
 
 // mode: 0 plain, 1 plain + sample log, 2 learned proposal (needs the
 // predicted masks of launch_features + the proposal network)
-hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, oslots, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, oslots, st);
-    else launch_orca_k<16>(s, maxo, mode, oslots, st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, oslots, n_cu, st);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, oslots, n_cu, st);
+    else launch_orca_k<16>(s, maxo, mode, oslots, n_cu, st);
     return hipGetLastError();
 }
 

@@ -279,9 +279,9 @@ class Simulator:
 
     def stats(self) -> dict:
         keys = ["steps", "agent_updates", "lp3d", "locate_fallback", "replans", "destroyed",
-                "obst_overflow", "learned_tried", "learned_accepted", "learned_proposed"]
-        a = (C.c_uint64 * 10)()
-        N.check(self._L.ecm_sim_stats(self._h, a, 10))
+                "obst_overflow", "learned_tried", "learned_accepted", "learned_proposed", "lp3d_big"]
+        a = (C.c_uint64 * len(keys))()
+        N.check(self._L.ecm_sim_stats(self._h, a, len(keys)))
         return {k: int(v) for k, v in zip(keys, a)}
 
     def set_timing(self, on: bool) -> None:

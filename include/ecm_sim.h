@@ -195,9 +195,11 @@ int ecm_sim_get_replans(ecm_sim* s, int* ids, int cap, int* n);
  * (3D LP run), [3] exact-y locate fallbacks, [4] replans, [5] destroyed,
  * [6] agents with more obstacle constraints than the main ORCA tile holds,
  * [7] agents the learned proposal was tried on, [8] of those, accepted by
- * the verification, [9] constraints proposed over the tried agents.
+ * the verification, [9] constraints proposed over the tried agents, [10] of
+ * the 3D-LP agents ([2]), those with more obstacle constraints than the
+ * 3D LP's register tile holds (solved from an LDS tile instead).
  * Copies min(n, ECM_SIM_N_STATS). */
-#define ECM_SIM_N_STATS 10
+#define ECM_SIM_N_STATS 11
 int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n);
 /* Device time of the phases of the last step in ms (HIP events on the
  * handle's stream): [0] grid build, [1] attraction + k-nearest, [2] learned

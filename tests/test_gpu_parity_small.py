@@ -84,6 +84,9 @@ def test_parity_pillar_forest_obstacle_overflow():
     st = sim.stats()
     sim.close()
     assert st["obst_overflow"] > 0, st
-    # the 3D LP ran on the device (k_lp3d) for the agents the oracle sent there
+    # the 3D LP ran on the device for the agents the oracle sent there, some
+    # of them with more obstacle constraints than its register tile holds
+    print(f"3D LP: {st['lp3d']} agents, {st['lp3d_big']} of them from the LDS tile", flush=True)
     assert st["lp3d"] > 0 and st["lp3d"] == used_3d, (st, used_3d)
+    assert st["lp3d_big"] > 0, st
     assert all(v == 0 for v in worst.values()), worst
