@@ -34,6 +34,11 @@ ECM_HD float det(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
 ECM_HD V2 right(V2 v) { return mk(v.y, -v.x); }
 ECM_HD V2 left(V2 v) { return mk(-v.y, v.x); }
 
+// std::min / std::max as the reference calls them on floats: (b < a) ? b : a
+// and (a < b) ? b : a, so a NaN second argument leaves the first unchanged
+ECM_HD float std_min(float a, float b) { return (b < a) ? b : a; }
+ECM_HD float std_max(float a, float b) { return (a < b) ? b : a; }
+
 // Vec2::Length / LengthSquared (ECMDataTypes.h:35-43)
 ECM_HD float len(V2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
 ECM_HD float len2(V2 v) { return v.x * v.x + v.y * v.y; }

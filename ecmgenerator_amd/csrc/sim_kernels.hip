@@ -17,6 +17,7 @@
 // One agent per lane; lanes are ordered along the Morton curve so a
 // wavefront covers a compact patch of the world.
 #include "ecm_geom.h"
+#include "lp_reg.h"
 #include "mlp.h"
 #include "sim_device.h"
 
@@ -928,8 +929,6 @@ __device__ __forceinline__ float4* q3d_slot(const DevState& s, uint32_t q, int c
 
 // Constraint::Contains, methodB (ORCA.h:49-60)
 __device__ __forceinline__ bool con_contains(const Con& c, V2 q) { return det(right(c.n), c.p - q) <= 0.0f; }
-__device__ __forceinline__ float std_min(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float std_max(float a, float b) { return (a < b) ? b : a; }
 
 // Obstacle half-plane for edge e (ORCA.cpp:70-333).  Returns false when the
 // reference emits nothing for it (`continue` without push_back).
@@ -1532,6 +1531,97 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
     orca_agent<K, OSLOTS, true, MODE>(s, t, lds, Tile{tid, OSLOTS + K, 0}, spill_used);
 }
 
+// Main ORCA pass of the plain step with every constraint in registers (no
+// LDS tile): up to OB obstacle constraints in a shift register (the last
+// nObst slots, in scan order) and the K agent constraints, RandomizedLP
+// over them (lp2_reg), then integration or the 3D-LP queue.  An agent with
+// more than OB obstacle constraints is parked untouched for k_orca_ovf.
+template <int K, int OB>
+__global__ __launch_bounds__(64) void k_orca_reg(DevState s) {
+    const uint32_t n_active = s.cell_start[s.n_keys];
+    const int t = xcd_block() * 64 + threadIdx.x;
+    if (t >= (int)n_active) return;
+    const float4 pr = s.sorted_pr[t];
+    const float4 sv = s.sorted_v[t];
+    const float2 v0 = make_float2(sv.x, sv.y);
+    if (s.dead_s[t]) {                   // destroyed this step: state unchanged
+        s.pos_s[t] = make_float2(pr.x, pr.y);
+        s.vel_s[t] = v0;
+        return;
+    }
+    const V2 position = mk(pr.x, pr.y);
+    const V2 velocity = mk(v0.x, v0.y);
+    const float rad = pr.z;
+    const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
+    const float2 pv = s.pref_vel_s[t];
+    const EnvDesc& E = env_of_sorted(s, sv);
+
+    V2 on[OB], op[OB];
+#pragma unroll
+    for (int u = 0; u < OB; ++u) { on[u] = mk(0.0f, 0.0f); op[u] = mk(0.0f, 0.0f); }
+    int nObst = 0;
+    if (E.n_obst_edges > 0) {
+        const float range = s.time_horizon_obst * maxSpeed + rad;          // ORCA.cpp:27
+        bool parked = false;
+        obstacle_query(E, position, range * range, [&](int e, float) {
+            V2 cp, cn;
+            if (!obstacle_constraint(s, E, e, position, velocity, rad, cp, cn)) return true;
+            if (nObst == OB) {
+                s.ovf_agent[wave_alloc(&s.stats[1])] = t;
+                parked = true;
+                return false;
+            }
+#pragma unroll
+            for (int u = 0; u + 1 < OB; ++u) { on[u] = on[u + 1]; op[u] = op[u + 1]; }
+            on[OB - 1] = cn;
+            op[OB - 1] = cp;
+            ++nObst;
+            return true;
+        });
+        if (parked) return;
+    }
+    const int cnt = s.nbr_cnt[t];
+    V2 an[K], ap[K];
+    uint32_t aval = 0u;
+    {
+        // all neighbour loads issued before the first constraint
+        float4 q[K];
+        float2 qv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int sj = j < cnt ? s.nbr[(size_t)j * s.n_slots + t] : t;
+            q[j] = s.sorted_pr[sj];
+            qv[j] = sorted_vel(s, sj);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            an[j] = mk(0.0f, 0.0f);
+            ap[j] = mk(0.0f, 0.0f);
+            if (j < cnt) {
+                agent_constraint(s, position, velocity, rad, q[j], qv[j], ap[j], an[j]);
+                aval |= 1u << j;
+            }
+        }
+    }
+    V2 o;
+    const int fs = lp2_reg<OB, K>(on, op, OB - nObst, an, ap, aval, mk(pv.x, pv.y), maxSpeed, o);
+    if (fs < OB + K) {
+        // RandomizedLP3D runs on the compacted queue (k_lp3d_reg)
+        const uint32_t qi = wave_alloc(&s.stats[0]);
+        s.q3d_agent[qi] = t;
+        s.q3d_info[qi] = make_int4(nObst + cnt, nObst, slot_to_index<OB>(fs, nObst), 0);
+        s.q3d_vel[qi] = make_float2(o.x, o.y);
+#pragma unroll
+        for (int u = 0; u < OB; ++u)
+            if (u >= OB - nObst) *q3d_slot(s, qi, u - (OB - nObst)) = make_float4(on[u].x, on[u].y, op[u].x, op[u].y);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < cnt) *q3d_slot(s, qi, nObst + j) = make_float4(an[j].x, an[j].y, ap[j].x, ap[j].y);
+        return;
+    }
+    integrate(s, t, position, velocity, o);
+}
+
 // Agents with more obstacle constraints than the main tile holds, with a tile for
 // MAXO (>= the longest obstacle bucket list), plain LP.  Fixed grid,
 // strided loop.
@@ -1573,110 +1663,6 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     }
 }
 
-// ORCA::RandomizedLP with useDirOpt (ORCA.cpp:428-587), the form the 3D LP
-// calls, over registers: the list is OB obstacle constraints (the first
-// nObst valid) followed by K bisectors (valid where bit j of pval is set),
-// in that order, fully unrolled so every constraint stays in a register.
-// Returns false where the reference returns a failing index.
-template <int OB, int K>
-__device__ __forceinline__ bool lp_dir_reg(const V2 (&on)[OB], const V2 (&op)[OB], int nObst, const V2 (&pn)[K],
-                                           const V2 (&pp)[K], uint32_t pval, V2 opt, float maxSpeed, V2& out) {
-    out = opt * maxSpeed;
-    bool ok = true;
-#pragma unroll
-    for (int i = 0; i < OB + K; ++i) {
-        const bool vi = i < OB ? (i < nObst) : ((pval >> (i - OB)) & 1u) != 0u;
-        const V2 hn = i < OB ? on[i < OB ? i : 0] : pn[i < OB ? 0 : i - OB];
-        const V2 hp = i < OB ? op[i < OB ? i : 0] : pp[i < OB ? 0 : i - OB];
-        if (ok && vi && det(right(hn), hp - out) > 0.0f) {
-            const V2 dir = right(hn);
-            const float dpd = dot(dir, hp);
-            const float disc = dpd * dpd + maxSpeed * maxSpeed - dot(hp, hp);
-            if (disc <= 0.0f) {
-                ok = false;
-            } else {
-                const float sq = sqrtf(disc);
-                float lft = -dpd - sq;
-                float rgt = -dpd + sq;
-                bool bad = false;
-#pragma unroll
-                for (int j = 0; j < i; ++j) {
-                    const bool vj = j < OB ? (j < nObst) : ((pval >> (j - OB)) & 1u) != 0u;
-                    const V2 jn = j < OB ? on[j < OB ? j : 0] : pn[j < OB ? 0 : j - OB];
-                    const V2 jp = j < OB ? op[j < OB ? j : 0] : pp[j < OB ? 0 : j - OB];
-                    if (!bad && vj) {
-                        const float den = det(right(hn), right(jn));
-                        const float num = det(right(jn), hp - jp);
-                        if (fabsf(den) <= kEps) {
-                            if (num < 0.0f) bad = true;
-                        } else {
-                            const float t = num / den;
-                            if (den >= 0.0f) rgt = std_min(rgt, t);
-                            else lft = std_max(lft, t);
-                            if (lft > rgt) bad = true;
-                        }
-                    }
-                }
-                if (bad) ok = false;
-                else if (dot(opt, right(hn)) > 0) out = hp + right(hn) * rgt;
-                else out = hp + right(hn) * lft;
-            }
-        }
-    }
-    return ok;
-}
-
-// ORCA::RandomizedLP3D (ORCA.cpp:592-669) over registers: OB obstacle
-// constraints (nObst valid) and K agent constraints (na valid), failing
-// index given as a position in that list (vfail).  The loop over the
-// candidate constraint runs over list positions, the same for every lane of
-// the wavefront, so the bisector count and the inner LP's length are
-// wave-uniform; a lane takes part from its failing position on.
-template <int OB, int K>
-__device__ __forceinline__ void lp3d_reg(const DevState& s, uint32_t q, const V2 (&on)[OB], const V2 (&op)[OB],
-                                         const V2 (&an)[K], const V2 (&ap)[K], int nObst, int na, int vfail,
-                                         float maxSpeed, V2& out) {
-    float maxPen = 0.0f;
-#pragma unroll 1
-    for (int v = 0; v < OB + K; ++v) {
-        const int jv = v - OB;
-        const bool in_list = v < OB ? (v < nObst) : (jv < na);
-        if (!(v >= vfail && in_list)) continue;
-        // the candidate constraint ci, read again from the queue entry: a
-        // register select by the loop position would index the arrays
-        const float4 c4 = *q3d_slot(s, q, v < OB ? v : nObst + jv);
-        const V2 cn = mk(c4.x, c4.y), cp = mk(c4.z, c4.w);
-        const V2 dir = right(cn);
-        if (!(det(dir, cp - out) > maxPen)) continue;
-        // bisectors with the agent constraints before ci
-        V2 pn[K], pp[K];
-        uint32_t pval = 0u;
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            pn[j] = mk(0.0f, 0.0f);
-            pp[j] = mk(0.0f, 0.0f);
-            if (j < jv) {
-                const float d = det(right(cn), right(an[j]));
-                if (fabsf(d) <= kEps) {
-                    if (!(dot(cn, an[j]) > 0)) {
-                        pp[j] = (cp + ap[j]) * 0.5f;
-                        pn[j] = normalized(an[j] - cn);
-                        pval |= 1u << j;
-                    }
-                } else {
-                    const float tt = det(right(an[j]), cp - ap[j]) / d;
-                    pp[j] = cp + right(cn) * tt;
-                    pn[j] = normalized(an[j] - cn);
-                    pval |= 1u << j;
-                }
-            }
-        }
-        const V2 temp = out;
-        if (!lp_dir_reg<OB, K>(on, op, nObst, pn, pp, pval, cn, maxSpeed, out)) out = temp;
-        maxPen = det(dir, cp - out);
-    }
-}
-
 // The 3D-LP queue (agents whose 2D LP failed), then integration: constraints
 // in registers (up to OB obstacle and K agent constraints; entries with more
 // obstacle constraints go to q3d_big for k_lp3d).  Each wavefront takes 64
@@ -1701,7 +1687,8 @@ __global__ __launch_bounds__(256) void k_lp3d_reg(DevState s) {
         V2 on[OB], op[OB], an[K], ap[K];
 #pragma unroll
         for (int c = 0; c < OB; ++c) {
-            const float4 v = c < nObst ? *q3d_slot(s, q, c) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // obstacle constraint c - (OB - nObst) in slot c (the last nObst slots)
+            const float4 v = c >= OB - nObst ? *q3d_slot(s, q, c - (OB - nObst)) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             on[c] = mk(v.x, v.y);
             op[c] = mk(v.z, v.w);
         }
@@ -1716,8 +1703,8 @@ __global__ __launch_bounds__(256) void k_lp3d_reg(DevState s) {
         const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
         const float2 ov = s.q3d_vel[q];
         V2 o = mk(ov.x, ov.y);
-        const int vfail = info.z < nObst ? info.z : OB + (info.z - nObst);
-        lp3d_reg<OB, K>(s, q, on, op, an, ap, nObst, na, vfail, maxSpeed, o);
+        lp3d_reg<OB, K>(on, op, an, ap, nObst, na, info.z, maxSpeed,
+                        [&](int c) { return *q3d_slot(s, q, c); }, o);
         integrate(s, t, mk(pr.x, pr.y), mk(sv.x, sv.y), o);
     }
 }
@@ -1807,11 +1794,15 @@ hipError_t launch_unsort(const DevState& s, hipStream_t st) {
 // Queue kernels run on a fixed grid: their queues hold a few percent of
 // the agents, and a full-size grid would reserve their LDS for nothing.
 constexpr int kQueueBlocks = 1024;
+// The plain step's main pass keeps its constraints in registers, with 4
+// obstacle slots (8 for OSLOTS 4 next to at most 10 agent constraints; 8
+// next to 16 would spill); the learned and logging passes use the LDS tile.
 template <int K, int OSLOTS>
 static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStream_t st) {
+    constexpr int OB = (OSLOTS == 4 && K <= 10) ? 8 : 4;
     if (mode == kLearned) hipLaunchKernelGGL((k_orca<K, kLearned, OSLOTS>), g, b, 0, st, s);
     else if (mode == kLog) hipLaunchKernelGGL((k_orca<K, kLog, OSLOTS>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_orca<K, kPlain, OSLOTS>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_orca_reg<K, OB>), g, b, 0, st, s);
 }
 
 // Obstacle constraints held in registers by k_lp3d_reg; its persistent

@@ -111,9 +111,12 @@ int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
 /* Launch shape; no choice changes any result.  cell_scale multiplies the
  * agent grid cell edge (about one agent per cell at 1; default 1.25),
  * knn_block is the workgroup size of the attraction + k-nearest kernel (64,
- * 128, 256; default 256), and orca_obst_slots the constraint slots per
- * agent of the main ORCA tile beyond k (1, 2, 4; default 1): agents with
- * more constraints borrow from their workgroup's small spill area, and
+ * 128, 256; default 256), and orca_obst_slots sizes the main ORCA pass (1,
+ * 2, 4; default 1): the plain step holds 4 obstacle constraints per agent
+ * in registers (8 with orca_obst_slots 4 and k <= 10) and sends agents with
+ * more to the overflow pass; the learned and logging steps
+ * give each agent orca_obst_slots slots beyond k in an LDS tile, agents
+ * with more constraints borrow from their workgroup's small spill area, and
  * those with more obstacle constraints than the tile holds, or whose
  * workgroup's spill area is used up, take the overflow pass.  best_ms: step
  * time the autotuner measured (0 before). */
