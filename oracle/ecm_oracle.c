@@ -392,6 +392,12 @@ static inline void push(con* cs, int* n, int cap, v2 p, v2 nrm) {
     ++*n;
 }
 
+/* Branches of the obstacle half-plane (ORCA.cpp:70-333) taken for the edges
+ * processed while g_obst_branches points somewhere (oracle_obstacle_halfplane);
+ * bits as ORACLE_OB_* in ecm_oracle.h. */
+static unsigned* g_obst_branches = NULL;
+#define OB_HIT(bit) do { if (g_obst_branches) *g_obst_branches |= (bit); } while (0)
+
 /* ORCA::GenerateConstraints (ORCA.cpp:60-424) */
 static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, const obst_soa* s,
                               int e, const int* obst, int n_obst_nb, const int* nbr, int n_nbr,
@@ -429,26 +435,31 @@ static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, 
         const float radiusSq = rad * rad;
 
         if (sp < 0.0f && distSq1 <= radiusSq) {              /* case 1 :92-105 */
+            OB_HIT(s->convex[L] ? ORACLE_OB_CASE1 : ORACLE_OB_CASE1_SKIP);
             if (s->convex[L]) push(cs, &nc, cap, V(0.0f, 0.0f), vnormalized(vmul(rp1, -1.0f)));
             continue;
         } else if (sp > 1.0f && distSq2 <= radiusSq) {       /* case 2 :108-121 */
             v2 rnd = vnormalize(vsub(s->p[s->next[R]], s->p[R]));
-            if (s->convex[R] && det(rp2, rnd) >= 0.0f)
-                push(cs, &nc, cap, V(0.0f, 0.0f), vnormalized(vmul(rp2, -1.0f)));
+            const int emit = s->convex[R] && det(rp2, rnd) >= 0.0f;
+            OB_HIT(emit ? ORACLE_OB_CASE2 : ORACLE_OB_CASE2_SKIP);
+            if (emit) push(cs, &nc, cap, V(0.0f, 0.0f), vnormalized(vmul(rp2, -1.0f)));
             continue;
         } else if (sp >= 0.0f && sp < 1.0f && distSqLine <= radiusSq) {  /* case 3 :124-135 */
+            OB_HIT(ORACLE_OB_CASE3);
             push(cs, &nc, cap, V(0.0f, 0.0f), right(seg));
             continue;
         }
 
         v2 leftLeg, rightLeg;
         if (sp < 0.0f && distSqLine <= radiusSq) {           /* :146-169 */
+            OB_HIT(s->convex[L] ? ORACLE_OB_VERTEX_L : ORACLE_OB_VERTEX_L_SKIP);
             if (!s->convex[L]) continue;
             R = L;
             const float leg1 = sqrtf(distSq1 - radiusSq);
             leftLeg = vdiv(V(rp1.x * leg1 - rp1.y * rad, rp1.x * rad + rp1.y * leg1), distSq1);
             rightLeg = vdiv(V(rp1.x * leg1 + rp1.y * rad, -rp1.x * rad + rp1.y * leg1), distSq1);
         } else if (sp > 1.0f && distSqLine <= radiusSq) {    /* :171-183 */
+            OB_HIT(s->convex[R] ? ORACLE_OB_VERTEX_R : ORACLE_OB_VERTEX_R_SKIP);
             if (!s->convex[R]) continue;
             L = R;
             const float leg2 = sqrtf(distSq2 - radiusSq);
@@ -459,12 +470,14 @@ static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, 
                 const float leg1 = sqrtf(distSq1 - radiusSq);
                 leftLeg = vdiv(V(rp1.x * leg1 - rp1.y * rad, rp1.x * rad + rp1.y * leg1), distSq1);
             } else {
+                OB_HIT(ORACLE_OB_CONCAVE_L);
                 leftLeg = vmul(seg, -1.0f);
             }
             if (s->convex[R]) {
                 const float leg2 = sqrtf(distSq2 - radiusSq);
                 rightLeg = vdiv(V(rp2.x * leg2 + rp2.y * rad, -rp2.x * rad + rp2.y * leg2), distSq2);
             } else {
+                OB_HIT(ORACLE_OB_CONCAVE_R);
                 rightLeg = seg;
             }
         }
@@ -473,11 +486,13 @@ static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, 
         int leftForeign = 0, rightForeign = 0;
         v2 lnd = vnormalize(vsub(s->p[L], s->p[s->prev[L]]));
         if (s->convex[L] && det(leftLeg, vmul(lnd, -1.0f)) >= 0.0f) {
+            OB_HIT(ORACLE_OB_FOREIGN_L);
             leftLeg = vmul(lnd, -1.0f);
             leftForeign = 1;
         }
         v2 rnd = vnormalize(vsub(s->p[s->next[R]], s->p[R]));
         if (s->convex[R] && det(rightLeg, rnd) <= 0.0f) {
+            OB_HIT(ORACLE_OB_FOREIGN_R);
             rightLeg = rnd;
             rightForeign = 1;
         }
@@ -492,11 +507,13 @@ static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, 
         const float tRight = dot(vsub(velocity, rightCutoff), rightLeg);
 
         if ((t < 0.0f && tLeft < 0.0f) || (same && tLeft < 0.0f && tRight < 0.0f)) {   /* :259-268 */
+            OB_HIT(ORACLE_OB_CIRCLE_L);
             v2 uw = vnormalized(vsub(velocity, leftCutoff));
             v2 p = vadd(leftCutoff, vmul(vmul(uw, invT), rad));
             push(cs, &nc, cap, p, uw);
             continue;
         } else if (t > 1.0f && tRight < 0.0f) {              /* :270-280 */
+            OB_HIT(ORACLE_OB_CIRCLE_R);
             v2 uw = vnormalized(vsub(velocity, rightCutoff));
             v2 p = vadd(rightCutoff, vmul(vmul(uw, invT), rad));
             push(cs, &nc, cap, p, uw);
@@ -508,17 +525,20 @@ static int gen_constraints_ex(const oracle_params* prm, const oracle_state* st, 
         const float dRight = (tRight < 0.0f) ? inf : vlen2(vsub(velocity, vadd(rightCutoff, vmul(rightLeg, tRight))));
 
         if (dCut <= dLeft && dCut <= dRight) {               /* :289-301 */
+            OB_HIT(ORACLE_OB_CUTOFF_LINE);
             v2 nrm = left(vmul(seg, -1.0f));
             v2 p = vadd(leftCutoff, vmul(vmul(nrm, invT), rad));
             push(cs, &nc, cap, p, nrm);
             continue;
         } else if (dLeft <= dRight) {                        /* :303-317 */
+            OB_HIT(leftForeign ? ORACLE_OB_LEG_L_SKIP : ORACLE_OB_LEG_L);
             if (leftForeign) continue;
             v2 nrm = left(leftLeg);
             v2 p = vadd(leftCutoff, vmul(vmul(nrm, rad), invT));
             push(cs, &nc, cap, p, nrm);
             continue;
         } else {                                             /* :319-332 */
+            OB_HIT(rightForeign ? ORACLE_OB_LEG_R_SKIP : ORACLE_OB_LEG_R);
             if (rightForeign) continue;
             v2 nrm = right(rightLeg);
             v2 p = vadd(rightCutoff, vmul(vmul(nrm, rad), invT));
@@ -935,6 +955,33 @@ int oracle_constraints(const oracle_params* prm, const oracle_state* st, const o
     }
     obst_free(&s);
     return nc;
+}
+
+int oracle_obstacle_halfplane(const oracle_params* prm, const oracle_state* st, const oracle_obstacles* obs, int agent,
+                              int edge, float* out4, unsigned* branches) {
+    obst_soa s;
+    if (obst_build(obs, &s) != 0) { obst_free(&s); return -1; }
+    if (edge < 0 || edge >= s.nv) { obst_free(&s); return -1; }
+    con c[2];
+    int nob = 0;
+    unsigned b = 0;
+    g_obst_branches = &b;
+    gen_constraints(prm, st, &s, agent, &edge, 1, NULL, 0, c, 2, &nob);
+    g_obst_branches = NULL;
+    if (nob > 0) { out4[0] = c[0].n.x; out4[1] = c[0].n.y; out4[2] = c[0].p.x; out4[3] = c[0].p.y; }
+    if (branches) *branches = b;
+    obst_free(&s);
+    return nob;
+}
+
+/* Simulator::FindNearestObstacles (Simulator.cpp:259-292): the edges kept for
+ * an agent at (x, y) with the given range, in scan order. */
+int oracle_nearest_obstacles(const oracle_obstacles* obs, float x, float y, float range, int* out, int cap) {
+    obst_soa s;
+    if (obst_build(obs, &s) != 0) { obst_free(&s); return -1; }
+    int n = nearest_obstacles(&s, V(x, y), range * range, out, cap);
+    obst_free(&s);
+    return n;
 }
 
 /* ------------------------------------------- learned-proposal samples */

@@ -121,6 +121,35 @@ int oracle_lp2d(const float* constraints, int n, const float* opt,
                 float max_speed, int use_dir_opt, float* out_vel);
 
 /* ORCA::RandomizedLP3D (ORCA.cpp:592-669). out_vel is in/out. */
+/* Obstacle half-plane (ORCA.cpp:70-333) of one edge (global vertex index
+ * `edge` -> its next vertex) for one agent: returns the number of
+ * constraints emitted (0 or 1), the constraint in out4 (nx, ny, px, py) and
+ * in *branches the ORACLE_OB_* bits of the branches taken. */
+#define ORACLE_OB_CASE1 (1u << 0)          /* collision with L, L convex (:92-99)         */
+#define ORACLE_OB_CASE1_SKIP (1u << 1)     /* collision with L, L concave (:102-104)      */
+#define ORACLE_OB_CASE2 (1u << 2)          /* collision with R, emitted (:108-117)        */
+#define ORACLE_OB_CASE2_SKIP (1u << 3)     /* collision with R, nothing (:119-120)        */
+#define ORACLE_OB_CASE3 (1u << 4)          /* collision with the segment (:124-135)       */
+#define ORACLE_OB_VERTEX_L (1u << 5)       /* oblique view of L alone (:146-169)          */
+#define ORACLE_OB_VERTEX_L_SKIP (1u << 6)  /* ... L concave (:149-151)                    */
+#define ORACLE_OB_VERTEX_R (1u << 7)       /* oblique view of R alone (:171-183)          */
+#define ORACLE_OB_VERTEX_R_SKIP (1u << 8)  /* ... R concave (:173-175)                    */
+#define ORACLE_OB_CONCAVE_L (1u << 9)      /* left leg along the segment, L concave (:193-199) */
+#define ORACLE_OB_CONCAVE_R (1u << 10)     /* right leg along the segment, R concave (:206-211) */
+#define ORACLE_OB_FOREIGN_L (1u << 11)     /* left leg clamped to the left neighbour (:224-228) */
+#define ORACLE_OB_FOREIGN_R (1u << 12)     /* right leg clamped to the right neighbour (:233-237) */
+#define ORACLE_OB_CIRCLE_L (1u << 13)      /* left cutoff circle (:259-268)               */
+#define ORACLE_OB_CIRCLE_R (1u << 14)      /* right cutoff circle (:270-280)              */
+#define ORACLE_OB_CUTOFF_LINE (1u << 15)   /* cutoff line (:289-301)                      */
+#define ORACLE_OB_LEG_L (1u << 16)         /* left leg (:303-317)                         */
+#define ORACLE_OB_LEG_L_SKIP (1u << 17)    /* ... foreign, nothing emitted (:305-307)     */
+#define ORACLE_OB_LEG_R (1u << 18)         /* right leg (:319-332)                        */
+#define ORACLE_OB_LEG_R_SKIP (1u << 19)    /* ... foreign, nothing emitted (:320-322)     */
+int oracle_obstacle_halfplane(const oracle_params* prm, const oracle_state* st, const oracle_obstacles* obs,
+                              int agent, int edge, float* out4, unsigned* branches);
+/* FindNearestObstacles (Simulator.cpp:259-292): edges kept, in scan order. */
+int oracle_nearest_obstacles(const oracle_obstacles* obs, float x, float y, float range, int* out, int cap);
+
 void oracle_lp3d(const float* constraints, int n, int n_obst, float max_speed,
                  int failed_index, float* out_vel);
 

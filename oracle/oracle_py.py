@@ -84,6 +84,12 @@ def lib():
         L.oracle_proposal_sample.restype = C.c_int
         L.oracle_proposal_sample.argtypes = [C.POINTER(OParams), C.POINTER(OState), C.POINTER(OObst), C.c_int,
                                              C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_obstacle_halfplane.restype = C.c_int
+        L.oracle_obstacle_halfplane.argtypes = [C.POINTER(OParams), C.POINTER(OState), C.POINTER(OObst), C.c_int,
+                                                C.c_int, C.c_void_p, C.POINTER(C.c_uint)]
+        L.oracle_nearest_obstacles.restype = C.c_int
+        L.oracle_nearest_obstacles.argtypes = [C.POINTER(OObst), C.c_float, C.c_float, C.c_float, C.c_void_p,
+                                               C.c_int]
         L.oracle_kdtree_build_only.restype = C.c_int
         L.oracle_kdtree_build_only.argtypes = [C.POINTER(OState)]
         _lib = L
@@ -218,6 +224,36 @@ def constraints(params, state, obstacles, agent, nbrs):
     n = lib().oracle_constraints(C.byref(P.prm), C.byref(P.st), C.byref(P.obs), int(agent), _p(nb),
                                  int(nb.shape[0]), _p(out), 256, _p(nob))
     return out[:n].copy(), int(nob[0])
+
+
+# branch bits of obstacle_halfplane (ORACLE_OB_* in ecm_oracle.h)
+OB_BITS = ["case1", "case1_skip", "case2", "case2_skip", "case3", "vertex_l", "vertex_l_skip", "vertex_r",
+           "vertex_r_skip", "concave_l", "concave_r", "foreign_l", "foreign_r", "circle_l", "circle_r",
+           "cutoff_line", "leg_l", "leg_l_skip", "leg_r", "leg_r_skip"]
+
+
+def obstacle_halfplane(params, state, obstacles, agent, edge):
+    """(constraints emitted (0/1), (nx, ny, px, py), set of branch names) of
+    obstacle edge `edge` for `agent` (ORCA.cpp:70-333)."""
+    P = Problem(params, state, obstacles, _EmptyEcm())
+    out = np.zeros(4, np.float32)
+    b = C.c_uint(0)
+    n = lib().oracle_obstacle_halfplane(C.byref(P.prm), C.byref(P.st), C.byref(P.obs), int(agent), int(edge), _p(out),
+                                        C.byref(b))
+    if n < 0:
+        raise RuntimeError("oracle_obstacle_halfplane failed")
+    return n, out, {name for i, name in enumerate(OB_BITS) if (b.value >> i) & 1}
+
+
+def nearest_obstacles(obstacles, x, y, range_):
+    """FindNearestObstacles (Simulator.cpp:259-292): kept edges in scan order."""
+    keep = [np.ascontiguousarray(obstacles.ring_off, np.int32), np.ascontiguousarray(obstacles.xy, np.float32)]
+    o = OObst(obstacles.n_rings, _p(keep[0]), _p(keep[1]))
+    out = np.zeros(4096, np.int32)
+    n = lib().oracle_nearest_obstacles(C.byref(o), float(x), float(y), float(range_), _p(out), out.shape[0])
+    if n < 0:
+        raise RuntimeError("oracle_nearest_obstacles failed")
+    return out[:min(n, out.shape[0])].copy()
 
 
 class _EmptyEcm:

@@ -132,6 +132,80 @@ def test_obstacle_halfplane_closed_form(oracle, case):
     assert np.allclose(cs[0, 2:], case["point"], atol=1e-6)
 
 
+def _shape_obstacles(ring):
+    from ecmgenerator_amd.simulator import Obstacles
+    xy = np.asarray(ring, np.float32)
+    return Obstacles(xy, np.array([0, xy.shape[0]], np.int32))
+
+
+@pytest.mark.parametrize("case", GOLD["obstacle_branches"],
+                         ids=lambda c: f"{c['shape']}{c['edge']}-{'+'.join(c['branches'])}")
+def test_obstacle_branch_geometry(oracle, case):
+    """Every branch of the obstacle half-plane (ORCA.cpp:91-332): the branch
+    the oracle takes and the half-plane it emits ({u : n . u >= n . p})
+    against the FP64 geometry of the truncated velocity obstacle
+    (tests/golden/make_golden.py, obstacle_model)."""
+    obs = _shape_obstacles(GOLD["shapes"][case["shape"]])
+    p = Params(k=1, time_horizon=case["T"], time_horizon_obst=case["T"])
+    st = _state([case["pos"]])
+    st["velocity"] = np.array([case["vel"]], np.float32)
+    st["clearance"] = np.array([case["r"]], np.float32)
+    n, c, branches = oracle.obstacle_halfplane(p, st, obs, 0, case["edge"])
+    assert branches == set(case["branches"]), (branches, case["branches"])
+    assert n == (1 if case["emitted"] else 0)
+    if case["emitted"]:
+        nrm = c[:2].astype(np.float64)
+        assert np.allclose(nrm, case["normal"], atol=1e-5), (nrm, case["normal"])
+        assert abs(float(nrm @ c[2:].astype(np.float64)) - case["offset"]) < 1e-5
+
+
+def test_obstacle_branch_cases_cover_every_branch():
+    hit = set().union(*(set(c["branches"]) for c in GOLD["obstacle_branches"]))
+    from oracle.oracle_py import OB_BITS
+    assert hit == set(OB_BITS)
+
+
+def test_obstacle_filter_brute_force(oracle):
+    """FindNearestObstacles (Simulator.cpp:259-292) against an FP64 brute
+    force: an edge is kept when the agent is strictly right of it (outside a
+    CCW ring) and the segment comes closer than the range; kept in scan order.
+    Random rings of squares, L shapes and triangles; points within 1e-3 of
+    either threshold are skipped (float rounding may decide those)."""
+    rng = np.random.default_rng(4)
+    rings = []
+    for i in range(30):
+        shape = list(GOLD["shapes"].values())[i % 3]
+        sc, ang = rng.uniform(0.5, 4.0), rng.uniform(0, 2 * np.pi)
+        off = rng.uniform(-20, 20, 2)
+        R = np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+        rings.append((np.asarray(shape) * sc) @ R.T + off)
+    from ecmgenerator_amd.simulator import Obstacles
+    xy = np.concatenate(rings).astype(np.float32)
+    off = np.cumsum([0] + [r.shape[0] for r in rings]).astype(np.int32)
+    obs = Obstacles(xy, off)
+    nxt = np.empty(xy.shape[0], np.int64)
+    for a, b in zip(off[:-1], off[1:]):
+        nxt[a:b] = np.roll(np.arange(a, b), -1)
+    A = xy.astype(np.float64)
+    B = A[nxt]
+    checked = kept_total = 0
+    for _ in range(600):
+        q = rng.uniform(-25, 25, 2)
+        rg = rng.uniform(2.0, 6.0)
+        sld = (A[:, 0] - q[0]) * (B[:, 1] - A[:, 1]) - (A[:, 1] - q[1]) * (B[:, 0] - A[:, 0])
+        ab = B - A
+        t = np.clip(np.einsum("ij,ij->i", q - A, ab) / np.einsum("ij,ij->i", ab, ab), 0, 1)
+        d = np.linalg.norm(q - (A + ab * t[:, None]), axis=1)
+        if np.any(np.abs(sld) < 1e-3) or np.any(np.abs(d - rg) < 1e-3):
+            continue
+        want = np.flatnonzero((sld < 0) & (d < rg))
+        got = oracle.nearest_obstacles(obs, np.float32(q[0]), np.float32(q[1]), np.float32(rg))
+        assert got.tolist() == want.tolist()
+        checked += 1
+        kept_total += want.size
+    assert checked > 400 and kept_total > 300
+
+
 def test_lp2d_matches_fp64_qp(oracle):
     rng = np.random.default_rng(1)
     checked = 0
