@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <atomic>
 #include <chrono>
 #include <memory>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -133,6 +135,7 @@ struct ecm_sim {
     DBuf<int> path_off, path_cnt;
     DBuf<float2> path_pool;
     size_t path_used = 0;
+    uint64_t path_compactions = 0;           // times compact_paths rebuilt the pool
     int cur = 0;
 
     // agent grid
@@ -782,9 +785,61 @@ extern "C" int ecm_sim_set_ecm(ecm_sim* s, int n_vert, const float* vert_xy, int
 }
 
 // ------------------------------------------------------------- agents
+// Spawns, replans and set_path append to the path pool; the points of
+// replaced paths and of destroyed agents stay behind (the reference frees
+// them, Simulator.cpp:100-105).  compact_paths rebuilds the pool with the
+// active agents' paths only, in slot order; the other slots get no path.
+static size_t live_path_points(const ecm_sim* s) {
+    size_t live = 0;
+    for (int i = 0; i < s->prm.max_agents; ++i)
+        if (s->h_active[i]) live += (size_t)std::max(0, s->h_path_cnt[i]);
+    return live;
+}
+
+static int compact_paths(ecm_sim* s) {
+    if (int rc = settle(s)) return rc;
+    const size_t N = (size_t)s->prm.max_agents;
+    std::vector<int> off(N), cnt(N);
+    std::vector<float2> pool(s->path_used), out;
+    HIPCHK(hipMemcpy(off.data(), s->path_off.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), s->path_cnt.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    if (s->path_used) HIPCHK(hipMemcpy(pool.data(), s->path_pool.p, s->path_used * sizeof(float2), hipMemcpyDeviceToHost));
+    out.reserve(live_path_points(s));
+    for (size_t i = 0; i < N; ++i) {
+        const bool keep = s->h_active[i] && cnt[i] > 0 && off[i] >= 0 && (size_t)off[i] + (size_t)cnt[i] <= s->path_used;
+        if (!keep) {
+            off[i] = 0;
+            cnt[i] = 0;
+            s->h_path_cnt[i] = 0;
+            continue;
+        }
+        const size_t o = out.size();
+        out.insert(out.end(), pool.begin() + off[i], pool.begin() + off[i] + cnt[i]);
+        off[i] = (int)o;
+    }
+    if (!out.empty()) HIPCHK(hipMemcpy(s->path_pool.p, out.data(), out.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->path_off.p, off.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->path_cnt.p, cnt.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    s->path_used = out.size();
+    ++s->path_compactions;
+    return 0;
+}
+
+// Room for `extra` more points.  Path offsets are int: the pool never
+// passes INT_MAX points.  Before growing, a pool whose dead points
+// outnumber the live ones is compacted instead.
 static int ensure_path_capacity(ecm_sim* s, size_t extra) {
+    const size_t kMaxPoints = (size_t)INT_MAX;
+    if (s->path_used + extra > kMaxPoints ||
+        (s->path_used + extra > s->path_pool.n && s->path_used > 2 * live_path_points(s) + extra)) {
+        int rc = compact_paths(s);
+        if (rc) return rc;
+    }
+    if (s->path_used + extra > kMaxPoints)
+        return fail(ECM_E_STATE, "path pool would pass %zu points (%zu live, %zu more asked)", kMaxPoints,
+                    s->path_used, extra);
     if (s->path_used + extra <= s->path_pool.n) return 0;
-    size_t cap = std::max(s->path_pool.n * 2, s->path_used + extra + 1024);
+    size_t cap = std::min(kMaxPoints, std::max(s->path_pool.n * 2, s->path_used + extra + 1024));
     float2* np = nullptr;
     if (hipMalloc((void**)&np, cap * sizeof(float2)) != hipSuccess) return fail(ECM_E_HIP, "path pool growth failed");
     if (s->path_used)
@@ -1269,18 +1324,23 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     std::vector<float2> pts;
     for (int i = 0; i < n; ++i) {
         if (!cnt[i]) continue;
-        int id = rid[i];
-        ids.push_back(id);
-        offs.push_back((int)(s->path_used + pts.size()));
+        ids.push_back(rid[i]);
+        offs.push_back((int)pts.size());
         cnts.push_back(cnt[i]);
         for (int q = 0; q < cnt[i]; ++q) pts.push_back(make_float2(res[i][2 * q], res[i][2 * q + 1]));
-        s->h_goal[id] = pts.back();
-        s->h_path_cnt[id] = cnt[i];
     }
     if (ids.empty()) return 0;
+    // the capacity call may compact the pool (and waits for the stream then):
+    // offsets are taken after it
     int rc = ensure_path_capacity(s, pts.size());
     if (rc) return rc;
     const size_t m = ids.size();
+    for (size_t j = 0; j < m; ++j) {
+        const int id = ids[j];
+        s->h_goal[id] = pts[(size_t)offs[j] + (size_t)cnts[j] - 1];
+        s->h_path_cnt[id] = cnts[j];
+        offs[j] += (int)s->path_used;
+    }
     // The pinned staging is reused from step to step without a stream
     // sync: wait for the copies of the last upload that read it (long done
     // in the usual case), and grow it only when the stream is idle.
@@ -1944,12 +2004,36 @@ extern "C" int ecm_sim_snapshot_save(ecm_sim* s, const char* path) {
     return 0;
 }
 
+static int snapshot_load_impl(ecm_sim* s, const char* path);
+
+// The file is untrusted: every count is bounded by the file's size and
+// every index checked before anything is allocated from it or sent to the
+// device, and nothing thrown inside crosses the C ABI.
 extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
     if (!s || !path) return fail(ECM_E_ARG, "null argument");
+    try {
+        return snapshot_load_impl(s, path);
+    } catch (const std::bad_alloc&) {
+        return fail(ECM_E_ARG, "%s: snapshot too large for this host", path);
+    } catch (...) {
+        return fail(ECM_E_ARG, "%s: unreadable snapshot", path);
+    }
+}
+
+static int snapshot_load_impl(ecm_sim* s, const char* path) {
     (void)hipSetDevice(s->dev);
     if (int rc_ = settle(s)) return rc_;
     FILE* f = fopen(path, "rb");
     if (!f) return fail(ECM_E_ARG, "cannot open %s", path);
+    uint64_t fsize = 0;
+    if (fseek(f, 0, SEEK_END) == 0) {
+        long e = ftell(f);
+        fsize = e > 0 ? (uint64_t)e : 0;
+    }
+    if (fseek(f, 0, SEEK_SET) != 0) {
+        fclose(f);
+        return fail(ECM_E_ARG, "%s: not seekable", path);
+    }
     uint32_t hdr[4];
     ecm_sim_params prm;
     int32_t ints[3];
@@ -1962,6 +2046,12 @@ extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
         return fail(ECM_E_ARG, "%s: not a snapshot for this handle (max_agents / k / environments differ)", path);
     }
     const size_t N = hdr[2];
+    // counts the rest of the file must be able to hold
+    if (pu > fsize / sizeof(float2) || pu > (uint64_t)INT_MAX || fs > N || (uint64_t)ints[2] > fsize / 32 ||
+        ints[0] < 0 || (size_t)ints[0] > N || ints[1] < -1 || ints[1] >= (int)N) {
+        fclose(f);
+        return fail(ECM_E_ARG, "%s: corrupt snapshot header", path);
+    }
     std::vector<float2> pos(N), vel(N), attr(N), pv(N), pool(pu);
     std::vector<float> rad(N), spd(N);
     std::vector<uint8_t> act(N);
@@ -1980,20 +2070,40 @@ extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
     std::vector<EnvData> ed(ints[2]);
     for (int e = 0; ok && e < ints[2]; ++e) {
         EnvData& d = ed[e];
-        ok = rd(f, d.box, 4) && rd(f, d.c, 4) && d.c[0] >= 1;
+        ok = rd(f, d.box, 4) && rd(f, d.c, 4) && d.c[0] >= 1 && d.c[1] >= 0 && d.c[2] >= 0 && d.c[3] >= 0 &&
+             (uint64_t)d.c[0] <= fsize / 4 && (uint64_t)d.c[1] <= fsize / 8 && (uint64_t)d.c[2] <= fsize / 40;
         if (!ok) break;
         d.ro.resize(d.c[0]);
-        ok = rd(f, d.ro.data(), d.ro.size());
-        if (!ok) break;
+        ok = rd(f, d.ro.data(), d.ro.size()) && d.ro[0] == 0;
+        for (size_t r = 1; ok && r < d.ro.size(); ++r) ok = d.ro[r] >= d.ro[r - 1] + 2;   // rings of >= 2 vertices
+        if (!ok || (uint64_t)d.ro.back() > fsize / 8) { ok = false; break; }
         d.oxy.resize(2 * (size_t)d.ro.back());
         d.vxy.resize(2 * (size_t)d.c[1]);
         d.tgt.resize(2 * (size_t)d.c[2]);
         d.pts.resize(8 * (size_t)d.c[2]);
         ok = rd(f, d.oxy.data(), d.oxy.size()) && rd(f, d.vxy.data(), d.vxy.size()) && rd(f, d.tgt.data(), d.tgt.size()) &&
              rd(f, d.pts.data(), d.pts.size());
+        for (size_t q = 0; ok && q < d.tgt.size(); ++q) ok = d.tgt[q] >= 0 && d.tgt[q] < d.c[1];
     }
     fclose(f);
-    if (!ok) return fail(ECM_E_ARG, "%s: truncated snapshot", path);
+    if (!ok) return fail(ECM_E_ARG, "%s: truncated or corrupt snapshot", path);
+    // every active slot has a path inside the pool and an environment of
+    // the file; the free stack holds distinct inactive slots
+    {
+        std::vector<uint8_t> on_stack(N, 0);
+        for (uint64_t i = 0; i < fs; ++i) {
+            const int id = stack[i];
+            if (id < 0 || (size_t)id >= N || act[id] || on_stack[id])
+                return fail(ECM_E_ARG, "%s: corrupt free-slot stack", path);
+            on_stack[id] = 1;
+        }
+        for (size_t i = 0; i < N; ++i) {
+            if (cnt[i] < 0 || off[i] < 0 || (uint64_t)off[i] + (uint64_t)cnt[i] > pu ||
+                (act[i] && (cnt[i] < 1 || henv[i] >= (uint16_t)ints[2])))
+                return fail(ECM_E_ARG, "%s: corrupt path or environment of slot %zu", path, i);
+        }
+    }
+    const ecm_sim_params prm_before = s->prm;
     s->prm = prm;
     int rc = 0;
     for (int e = 0; !rc && e < ints[2]; ++e) {
@@ -2014,7 +2124,10 @@ extern "C" int ecm_sim_snapshot_load(ecm_sim* s, const char* path) {
         E->grid_for_n = -1;
     }
     if (!rc) rc = ensure_path_capacity(s, pu > s->path_used ? pu - s->path_used : 0);
-    if (rc) return rc;
+    if (rc) {
+        s->prm = prm_before;
+        return rc;
+    }
     (void)hipSetDevice(s->dev);
     if (pu) HIPCHK(hipMemcpy(s->path_pool.p, pool.data(), pu * sizeof(float2), hipMemcpyHostToDevice));
     s->path_used = pu;
