@@ -92,3 +92,62 @@ def test_planner_paths_are_valid():
         assert np.allclose(p[0], ag.start[i]) and np.allclose(p[-1], go[i])
         # path vertices never inside obstacles
         assert not any(sc.InsideObstacle(*q) for q in p)
+
+
+def _edge_arrays(obs):
+    xy = obs.xy.astype(np.float64)
+    a, b = [], []
+    for lo, hi in zip(obs.ring_off[:-1], obs.ring_off[1:]):
+        for i in range(lo, hi):
+            a.append(xy[i])
+            b.append(xy[lo + (i - lo + 1) % (hi - lo)])
+    return np.array(a), np.array(b)
+
+
+def _dist_to_obstacles(q, A, B):
+    """FP64 distance from the points q [m, 2] to the nearest obstacle edge."""
+    ab = B - A
+    out = np.empty(q.shape[0])
+    for k, x in enumerate(q):
+        t = np.clip(np.einsum("ij,ij->i", x - A, ab) / np.einsum("ij,ij->i", ab, ab), 0.0, 1.0)
+        out[k] = np.linalg.norm(x - (A + ab * t[:, None]), axis=1).min()
+    return out
+
+
+@pytest.mark.parametrize("config,n", [(0, 400), (1, 250)])
+def test_planner_paths_keep_the_corridor_clearance(config, n):
+    """What ECMPathPlanner::FindPath (ECMPathPlanner.cpp:22-136) guarantees
+    for an agent of clearance r, checked in FP64 against every obstacle
+    edge:
+    * the path runs from the start to the goal, exactly;
+    * every interior vertex is a point of the corridor shrunk by r
+      (ShrinkCorridor, :159-215), so no obstacle is closer than r to it;
+    * a segment between two interior vertices stays at least r cos 45 deg
+      from the obstacles: it is a chord of a shrunk-corridor arc around a
+      rectangle corner (at most 90 deg, sampled at its ends when shorter
+      than 10 m, SampleCorridorArc :255-279) or lies inside the corridor;
+    * no segment, including the first and last (FitPortalRange, :281-321,
+      drops the portals before the start and past the goal), touches an
+      obstacle, and no point of the path is inside one."""
+    sc, ag, go = SC.generate(config, seed=5, n_agents=n)
+    A, B = _edge_arrays(sc.obstacles())
+    worst = {"vertex": np.inf, "inner": np.inf, "end": np.inf}
+    for i in range(ag.n):
+        p = ag.path_xy[ag.path_off[i]:ag.path_off[i] + ag.path_cnt[i]].astype(np.float64)
+        r = float(ag.radius[i])
+        assert np.array_equal(p[0], ag.start[i]) and np.array_equal(p[-1], go[i])
+        if p.shape[0] > 2:
+            worst["vertex"] = min(worst["vertex"], _dist_to_obstacles(p[1:-1], A, B).min() / r)
+        for k in range(p.shape[0] - 1):
+            t = np.linspace(0.0, 1.0, 65)[:, None]
+            samples = p[k] + (p[k + 1] - p[k]) * t
+            d = _dist_to_obstacles(samples, A, B).min() / r
+            inner = 0 < k < p.shape[0] - 2
+            worst["inner" if inner else "end"] = min(worst["inner" if inner else "end"], d)
+            if k % 2 == 0:
+                assert not any(sc.InsideObstacle(*q) for q in samples[::8])
+    print(f"c{config}: closest obstacle / r: interior vertices {worst['vertex']:.5f}, inner segments "
+          f"{worst['inner']:.4f}, first and last segments {worst['end']:.4f}", flush=True)
+    assert worst["vertex"] >= 1.0 - 1e-3
+    assert worst["inner"] >= np.cos(np.pi / 4) * (1.0 - 1e-3)
+    assert worst["end"] > 0.2
