@@ -1268,6 +1268,14 @@ static DevState make_state(ecm_sim* s) {
     d.q3d_cap = (uint32_t)s->prm.max_agents; d.q3d_big = s->q3d_big.p;
     d.k = s->prm.k;
     d.time_horizon = s->prm.time_horizon;
+    {
+        // x / T == x * (1 / T) bit for bit when 1 / T is exact: both round
+        // the same real number (T a power of two, 1 / T not subnormal)
+        int e = 0;
+        const float m = std::frexp(s->prm.time_horizon, &e);
+        const float inv = 1.0f / s->prm.time_horizon;
+        d.inv_time_horizon = (m == 0.5f && std::isnormal(inv) && inv * s->prm.time_horizon == 1.0f) ? inv : 0.0f;
+    }
     d.time_horizon_obst = s->prm.time_horizon_obst;
     d.inv_time_horizon_obst = 1.0f / s->prm.time_horizon_obst;
     d.max_speed = s->prm.max_speed;

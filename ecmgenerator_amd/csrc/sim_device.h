@@ -161,6 +161,7 @@ struct DevState {
     // ---- parameters (ORCA.h:85-103, Simulator.cpp:538-686) ----
     int k;
     float time_horizon;
+    float inv_time_horizon;       // 1 / time_horizon when that is exact (a power of two), else 0
     float time_horizon_obst;
     float inv_time_horizon_obst;  // 1.0f / time_horizon_obst (ORCA.cpp:241)
     float max_speed;

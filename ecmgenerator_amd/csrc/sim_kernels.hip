@@ -1048,10 +1048,18 @@ __device__ bool obstacle_constraint(const DevState& s, const EnvDesc& E, int e, 
 __device__ void agent_constraint(const DevState& s, V2 position, V2 velocity, float rad,
                                  float4 q, float2 qv, V2& outP, V2& outN) {
     const float T = s.time_horizon;
-    V2 VOPos = mk((q.x - position.x) / T, (q.y - position.y) / T);
+    const float combined = q.z + rad;
+    V2 VOPos;
+    float VORadius;
+    if (s.inv_time_horizon != 0.0f) {           // T a power of two: the same values without dividing
+        const float iT = s.inv_time_horizon;
+        VOPos = mk((q.x - position.x) * iT, (q.y - position.y) * iT);
+        VORadius = combined * iT;
+    } else {
+        VOPos = mk((q.x - position.x) / T, (q.y - position.y) / T);
+        VORadius = combined / T;
+    }
     float VOPosLength = len(VOPos);
-    float combined = q.z + rad;
-    float VORadius = combined / T;
     V2 relVel = mk(velocity.x - qv.x, velocity.y - qv.y);
     V2 relPos = mk(q.x - position.x, q.y - position.y);
     float relPosLength = len(relPos);
@@ -1669,7 +1677,7 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
 // consecutive entries (coalesced slot-major loads); wavefronts are dealt to
 // the blocks round robin so that a short queue still spreads over the chip.
 template <int K, int OB>
-__global__ __launch_bounds__(256) void k_lp3d_reg(DevState s) {
+__global__ __launch_bounds__(256, 4) void k_lp3d_reg(DevState s) {
     const uint32_t nq = s.stats[0];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wpb = blockDim.x >> 6;
