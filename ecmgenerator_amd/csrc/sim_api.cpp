@@ -37,8 +37,10 @@ hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, cons
                                 const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
                                 float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
                                 int* path_cnt, hipStream_t st);
-hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt,
-                               hipStream_t st);
+hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
+                               int* path_cnt, int2* path_s, hipStream_t st);
+hipError_t launch_path_resort(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
+                              const int* path_off, const int* path_cnt, int2* path_s_old, hipStream_t st);
 }  // namespace ecm
 
 using namespace ecm;
@@ -145,7 +147,9 @@ struct ecm_sim {
     DBuf<float2> attr_s[2], pos_s[2], vel_s[2];
     DBuf<uint8_t> dead_s[2];
     DBuf<float2> pref_vel_s[2], v_orca_s[2];
+    DBuf<int2> path_s[2];                       // path (offset, count) by sorted position
     bool sorted_valid = false;                  // the last step's sorted outputs hold the state
+    bool paths_resort = false;                  // the host rewrote path_off / path_cnt since the last step
     bool id_current = true;                     // the slot-indexed arrays hold it
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
     DBuf<int> fwd, prev_of, pos_of;    // previous <-> this step's sorted positions; id -> last one
@@ -278,7 +282,7 @@ static void free_all(ecm_sim* s) {
     s->attraction.release(); s->pref_vel.release(); s->v_orca.release();
     for (int b = 0; b < 2; ++b) {
         s->sorted_pr[b].release(); s->sorted_v[b].release(); s->attr_s[b].release();
-        s->pos_s[b].release(); s->vel_s[b].release(); s->dead_s[b].release();
+        s->pos_s[b].release(); s->vel_s[b].release(); s->dead_s[b].release(); s->path_s[b].release();
     }
     for (int b = 0; b < 2; ++b) { s->pref_vel_s[b].release(); s->v_orca_s[b].release(); }
     s->radius.release(); s->pref_speed.release(); s->active.release();
@@ -385,6 +389,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     for (int b = 0; b < 2 && !rc; ++b) {
         rc |= s->sorted_pr[b].alloc(N); rc |= s->sorted_v[b].alloc(N); rc |= s->attr_s[b].alloc(N);
         rc |= s->pos_s[b].alloc(N); rc |= s->vel_s[b].alloc(N); rc |= s->dead_s[b].alloc(N);
+        rc |= s->path_s[b].alloc(N);
     }
     for (int b = 0; b < 2 && !rc; ++b) {
         rc |= s->pref_vel_s[b].alloc(N);
@@ -822,6 +827,7 @@ static int compact_paths(ecm_sim* s) {
     HIPCHK(hipMemcpy(s->path_cnt.p, cnt.data(), N * sizeof(int), hipMemcpyHostToDevice));
     s->path_used = out.size();
     ++s->path_compactions;
+    s->paths_resort = true;        // the last step's sorted copy holds the old offsets
     return 0;
 }
 
@@ -996,6 +1002,7 @@ extern "C" int ecm_sim_spawn(ecm_sim* s, int n, const float* start_xy, const flo
 // sorted-entry build are dropped (the slot arrays already hold them).
 static void invalidate_sorted(ecm_sim* s) {
     s->sorted_valid = false;
+    s->paths_resort = false;
     for (int id : s->pending_new) s->h_pending[id] = 0;
     s->pending_new.clear();
     s->pending_dead.clear();
@@ -1013,6 +1020,11 @@ static int upload_pending(ecm_sim* s) {
         return 0;
     }
     const int o = s->cur ^ 1;
+    if (s->paths_resort) {
+        HIPCHK(launch_path_resort(s->prm.max_agents, s->acc.p, s->sorted_pr[o].p, s->path_off.p, s->path_cnt.p,
+                                  s->path_s[o].p, s->stream));
+        s->paths_resort = false;
+    }
     if (!s->pending_dead.empty()) {
         s->dead_stage.swap(s->pending_dead);
         s->pending_dead.clear();
@@ -1088,6 +1100,7 @@ extern "C" int ecm_sim_set_path(ecm_sim* s, int id, const float* xy, int n) {
     s->path_used += (size_t)n;
     s->h_goal[id] = make_float2(xy[2 * (n - 1)], xy[2 * (n - 1) + 1]);
     s->h_path_cnt[id] = n;
+    s->paths_resort = true;
     return 0;
 }
 
@@ -1232,9 +1245,10 @@ static DevState make_state(ecm_sim* s) {
     d.sorted_pr_old = s->sorted_pr[o].p; d.sorted_v_old = s->sorted_v[o].p;
     d.pos_s_old = s->pos_s[o].p; d.vel_s_old = s->vel_s[o].p;
     d.attr_s_old = s->attr_s[o].p; d.dead_s_old = s->dead_s[o].p;
-    d.pref_vel_s_old = s->pref_vel_s[o].p; d.v_orca_s_old = s->v_orca_s[o].p;
+    d.pref_vel_s_old = s->pref_vel_s[o].p; d.v_orca_s_old = s->v_orca_s[o].p; d.path_s_old = s->path_s[o].p;
     d.attr_s = s->attr_s[s->cur].p; d.pref_vel_s = s->pref_vel_s[s->cur].p; d.dead_s = s->dead_s[s->cur].p;
     d.pos_s = s->pos_s[s->cur].p; d.vel_s = s->vel_s[s->cur].p; d.v_orca_s = s->v_orca_s[s->cur].p;
+    d.path_s = s->path_s[s->cur].p;
     d.n_env = (int)s->envs.size();
     d.envs = s->d_envs.p;
     d.agent_env = s->envs.size() > 1 ? s->agent_env.p : nullptr;
@@ -1301,7 +1315,8 @@ static int sync_state(ecm_sim* s) {
 // failed: planned from their step-start position to their goal.  Runs on
 // replan_threads threads while the GPU finishes the step; the new paths
 // are uploaded stream-ordered, before the next step reads them.
-static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vector<float2>& rpos) {
+static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vector<int>& rat,
+                       const std::vector<float2>& rpos) {
     const int n = (int)rid.size();
     const int cap = 4096;
     std::vector<std::vector<float>> res(n);
@@ -1328,11 +1343,12 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
         for (int i; (i = next++) < n;) work(i);
         for (auto& t : th) t.join();
     }
-    std::vector<int> ids, offs, cnts;
+    std::vector<int> ids, offs, cnts, ats;
     std::vector<float2> pts;
     for (int i = 0; i < n; ++i) {
         if (!cnt[i]) continue;
         ids.push_back(rid[i]);
+        ats.push_back(rat[i]);
         offs.push_back((int)pts.size());
         cnts.push_back(cnt[i]);
         for (int q = 0; q < cnt[i]; ++q) pts.push_back(make_float2(res[i][2 * q], res[i][2 * q + 1]));
@@ -1353,12 +1369,12 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     // sync: wait for the copies of the last upload that read it (long done
     // in the usual case), and grow it only when the stream is idle.
     HIPCHK(hipEventSynchronize(s->ev_stage));
-    if (s->h_stage_cap < 3 * m || s->h_pts_cap < pts.size() || s->d_stage.n < 3 * m) {
+    if (s->h_stage_cap < 4 * m || s->h_pts_cap < pts.size() || s->d_stage.n < 4 * m) {
         if ((rc = settle(s)) != 0) return rc;
-        if (s->h_stage_cap < 3 * m) {
+        if (s->h_stage_cap < 4 * m) {
             if (s->h_stage) (void)hipHostFree(s->h_stage);
             s->h_stage = nullptr;
-            s->h_stage_cap = std::max<size_t>(3 * m, 3 * 1024);
+            s->h_stage_cap = std::max<size_t>(4 * m, 4 * 1024);
             if (hipHostMalloc((void**)&s->h_stage, s->h_stage_cap * sizeof(int)) != hipSuccess) {
                 s->h_stage_cap = 0;
                 return fail(ECM_E_HIP, "hipHostMalloc failed");
@@ -1373,18 +1389,21 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
                 return fail(ECM_E_HIP, "hipHostMalloc failed");
             }
         }
-        if (s->d_stage.n < 3 * m && (rc = s->d_stage.alloc(std::max<size_t>(3 * m, 3 * 1024))) != 0) return rc;
+        if (s->d_stage.n < 4 * m && (rc = s->d_stage.alloc(std::max<size_t>(4 * m, 4 * 1024))) != 0) return rc;
     }
     std::memcpy(s->h_stage, ids.data(), m * sizeof(int));
     std::memcpy(s->h_stage + m, offs.data(), m * sizeof(int));
     std::memcpy(s->h_stage + 2 * m, cnts.data(), m * sizeof(int));
+    std::memcpy(s->h_stage + 3 * m, ats.data(), m * sizeof(int));
     std::memcpy(s->h_pts, pts.data(), pts.size() * sizeof(float2));
     HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, s->h_pts, pts.size() * sizeof(float2), hipMemcpyHostToDevice,
                           s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_stage.p, s->h_stage, 3 * m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_stage.p, s->h_stage, 4 * m * sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev_stage, s->stream));
-    HIPCHK(launch_path_scatter((int)m, s->d_stage.p, s->d_stage.p + m, s->d_stage.p + 2 * m, s->path_off.p,
-                               s->path_cnt.p, s->stream));
+    // the step that asked for them is this one ([cur]): its sorted copy is
+    // what the next step carries in
+    HIPCHK(launch_path_scatter((int)m, s->d_stage.p, s->d_stage.p + m, s->d_stage.p + 2 * m, s->d_stage.p + 3 * m,
+                               s->path_off.p, s->path_cnt.p, s->path_s[s->cur].p, s->stream));
     s->path_used += pts.size();
     return 0;
 }
@@ -1569,11 +1588,16 @@ static int step_once(ecm_sim* s) {
         HIPCHK(hipMemcpy(ev.data() + peek, s->xfer.p + 8 + 4 * (size_t)peek, (ne - peek) * sizeof(int4),
                          hipMemcpyDeviceToHost));
     }
-    std::vector<int> rid, dead;
+    // (id, x, y, sorted position) asks for a replan, (id, 0, 0, -1) is a destroy
+    std::vector<int> rid, rat, dead;
     std::vector<float2> rpos;
     for (const int4& e : ev) {
-        if (e.w) dead.push_back(e.x);
-        else { rid.push_back(e.x); rpos.push_back(make_float2(bits_to_float(e.y), bits_to_float(e.z))); }
+        if (e.w < 0) dead.push_back(e.x);
+        else {
+            rid.push_back(e.x);
+            rat.push_back(e.w);
+            rpos.push_back(make_float2(bits_to_float(e.y), bits_to_float(e.z)));
+        }
     }
     const uint32_t nr = (uint32_t)rid.size(), nd = (uint32_t)dead.size();
     // order as the reference visits slots (Simulator.cpp:545)
@@ -1581,17 +1605,18 @@ static int step_once(ecm_sim* s) {
         std::vector<int> ord(nr);
         for (uint32_t i = 0; i < nr; ++i) ord[i] = (int)i;
         std::sort(ord.begin(), ord.end(), [&](int a, int b) { return rid[a] < rid[b]; });
-        std::vector<int> r2(nr);
+        std::vector<int> r2(nr), a2(nr);
         std::vector<float2> p2(nr);
-        for (uint32_t i = 0; i < nr; ++i) { r2[i] = rid[ord[i]]; p2[i] = rpos[ord[i]]; }
+        for (uint32_t i = 0; i < nr; ++i) { r2[i] = rid[ord[i]]; a2[i] = rat[ord[i]]; p2[i] = rpos[ord[i]]; }
         rid.swap(r2);
+        rat.swap(a2);
         rpos.swap(p2);
     }
     s->h_replans = rid;
     bool any_planner = false;
     for (auto& e : s->envs) any_planner |= e->replan_fn != nullptr;
     if (nr && any_planner) {
-        rc = run_replans(s, rid, rpos);
+        rc = run_replans(s, rid, rat, rpos);
         if (rc) return rc;
     }
     // The step returns with its ORCA kernels possibly still running: every

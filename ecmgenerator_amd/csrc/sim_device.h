@@ -84,6 +84,7 @@ struct DevState {
     const uint8_t* dead_s_old;     // destroyed during it (its active count is acc[13])
     const float2* pref_vel_s_old;  // its preferred velocities
     const float2* v_orca_s_old;    // its ORCA velocities (may be null)
+    const int2* path_s_old;        // its path (offset, count) per agent
 
     // ---- this step's state by sorted position ----
     float2* attr_s;                // attraction point (carried in, updated by k_attr_knn)
@@ -92,6 +93,7 @@ struct DevState {
     float2* pos_s;                 // end-of-step position
     float2* vel_s;                 // end-of-step velocity
     float2* v_orca_s;              // ORCA output velocity (diagnostic, may be null)
+    int2* path_s;                  // path (offset, count): path_off / path_cnt carried in Morton order
 
     // ---- environments ----
     int n_env;

@@ -239,6 +239,7 @@ __global__ void k_grid_scatter(DevState s) {
     s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[i], __int_as_float(i));
     s.sorted_v[slot] = make_float4(v.x, v.y, s.pref_speed[i], __int_as_float(slot_env(s, i)));
     s.attr_s[slot] = s.attraction[i];
+    s.path_s[slot] = make_int2(s.path_off[i], s.path_cnt[i]);
     const int u = s.pos_of[i];
     const bool was = u >= 0 && u < (int)(uint32_t)s.acc[14] && __float_as_int(s.sorted_pr_old[u].w) == i;
     s.prev_of[slot] = was ? u : -1;
@@ -262,6 +263,7 @@ __global__ void k_grid_scatter_sorted(DevState s) {
         s.sorted_pr[slot] = make_float4(p.x, p.y, s.radius[id], __int_as_float(id));
         s.sorted_v[slot] = make_float4(v.x, v.y, s.pref_speed[id], __int_as_float(slot_env(s, id)));
         s.attr_s[slot] = s.attraction[id];
+        s.path_s[slot] = make_int2(s.path_off[id], s.path_cnt[id]);
         s.prev_of[slot] = -1;
         return;
     }
@@ -270,6 +272,7 @@ __global__ void k_grid_scatter_sorted(DevState s) {
     s.sorted_pr[slot] = make_float4(p.x, p.y, pr.z, pr.w);
     s.sorted_v[slot] = make_float4(v.x, v.y, sv.z, sv.w);
     s.attr_s[slot] = s.attr_s_old[t];
+    s.path_s[slot] = s.path_s_old[t];
     s.fwd[t] = (int)slot;
     s.prev_of[slot] = t;
 }
@@ -785,13 +788,14 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     if (t >= (int)n_active) return false;
     float4 pr = s.sorted_pr[t];
     const float4 sv = s.sorted_v[t];
+    const int2 pc = s.path_s[t];
     const int id = __float_as_int(pr.w);
     const V2 pos = mk(pr.x, pr.y);
     const EnvDesc& E = env_of_sorted(s, sv);
 
     // UpdateAttractionPointSystem (Simulator.cpp:538-590)
-    const int off = s.path_off[id], np = s.path_cnt[id];
-    const float2* path = s.path_pts + off;
+    const int np = pc.y;
+    const float2* path = s.path_pts + pc.x;
     float2 gg = path[np - 1];
     float gx = pos.x - gg.x, gy = pos.y - gg.y;
     float dGoal = gx * gx + gy * gy;
@@ -808,7 +812,7 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
         else {
             // UpdatePath(e, currentPosition, goal) with the step-start position
             uint32_t q = atomicAdd(s.events_n, 1u);
-            s.events[q] = make_int4(id, __float_as_int(pos.x), __float_as_int(pos.y), 0);
+            s.events[q] = make_int4(id, __float_as_int(pos.x), __float_as_int(pos.y), t);
             event = true;
         }
     }
@@ -830,7 +834,7 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
         }
         s.active[id] = 0;                                   // DestroyAgent
         uint32_t q = atomicAdd(s.events_n, 1u);
-        s.events[q] = make_int4(id, 0, 0, 1);
+        s.events[q] = make_int4(id, 0, 0, -1);
         s.nbr_cnt[t] = 0;
         __threadfence();
         return true;
@@ -1900,18 +1904,41 @@ __global__ void k_spawn_scatter(int n, const int* ids, const float2* start, cons
     path_cnt[id] = cnt[i];
 }
 
-// UpdatePath results for a batch of agents (Simulator.cpp:114-123)
-__global__ void k_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt) {
+// UpdatePath results for a batch of agents (Simulator.cpp:114-123).  With
+// `at` (each agent's position in the last step's sorted outputs), the copy
+// the next step carries in Morton order is updated too.
+__global__ void k_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
+                               int* path_cnt, int2* path_s) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     path_off[ids[i]] = off[i];
     path_cnt[ids[i]] = cnt[i];
+    if (at) path_s[at[i]] = make_int2(off[i], cnt[i]);
 }
 
-hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, int* path_off, int* path_cnt,
-                               hipStream_t st) {
+hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
+                               int* path_cnt, int2* path_s, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_path_scatter, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ids, off, cnt, path_off, path_cnt);
+    hipLaunchKernelGGL(k_path_scatter, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ids, off, cnt, at, path_off, path_cnt,
+                       path_s);
+    return hipGetLastError();
+}
+
+// The last step's sorted copy of the paths, re-read from the slot arrays
+// after the host rewrote them (set_path, pool compaction).
+__global__ void k_path_resort(const unsigned long long* acc, const float4* sorted_pr_old, const int* path_off,
+                              const int* path_cnt, int2* path_s_old) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)(uint32_t)acc[13]) return;           // the last step's active count
+    const int id = __float_as_int(sorted_pr_old[t].w);
+    path_s_old[t] = make_int2(path_off[id], path_cnt[id]);
+}
+
+hipError_t launch_path_resort(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
+                              const int* path_off, const int* path_cnt, int2* path_s_old, hipStream_t st) {
+    if (n_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_path_resort, dim3(cdiv(n_slots, 256)), dim3(256), 0, st, acc, sorted_pr_old, path_off,
+                       path_cnt, path_s_old);
     return hipGetLastError();
 }
 
