@@ -39,6 +39,9 @@ hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, cons
                                 int* path_cnt, hipStream_t st);
 hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
                                int* path_cnt, int2* path_s, hipStream_t st);
+hipError_t launch_path_repack(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
+                              const uint8_t* dead_s_old, int2* path_s_old, int* path_off, const float2* pool,
+                              float2* out, uint32_t cap, uint32_t* ctr, hipStream_t st);
 hipError_t launch_path_resort(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
                               const int* path_off, const int* path_cnt, int2* path_s_old, hipStream_t st);
 }  // namespace ecm
@@ -138,6 +141,15 @@ struct ecm_sim {
     DBuf<float2> path_pool;
     size_t path_used = 0;
     uint64_t path_compactions = 0;           // times compact_paths rebuilt the pool
+    // Morton-order repacks of the path pool (repack_paths): the second pool
+    // they pack into, the packed total counted on the device and read back
+    // into pinned memory, the totals the host expected
+    DBuf<float2> path_alt;
+    DBuf<uint32_t> repack_ctr;
+    uint32_t* h_repack = nullptr;
+    uint64_t repack_expect = 0;
+    bool repack_pending_check = false;
+    int steps_since_repack = 0;
     int cur = 0;
 
     // agent grid
@@ -298,6 +310,10 @@ static void free_all(ecm_sim* s) {
     s->agent_env.release();
     if (s->h_xfer) (void)hipHostFree(s->h_xfer);
     s->h_xfer = nullptr;
+    if (s->h_repack) (void)hipHostFree(s->h_repack);
+    s->h_repack = nullptr;
+    s->path_alt.release();
+    s->repack_ctr.release();
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     if (s->h_pts) (void)hipHostFree(s->h_pts);
     s->h_stage = nullptr;
@@ -423,6 +439,9 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
     if (!rc && hipHostGetDevicePointer((void**)&s->h_xfer_dev, s->h_xfer, 0) != hipSuccess)
         rc = fail(ECM_E_HIP, "hipHostGetDevicePointer failed");
     if (!rc) s->h_xfer[0] = 0u;
+    if (!rc) rc |= s->repack_ctr.alloc(2);
+    if (!rc && hipHostMalloc((void**)&s->h_repack, 2 * sizeof(uint32_t)) != hipSuccess)
+        rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipEventCreateWithFlags(&s->ev_stage, hipEventDisableTiming) != hipSuccess)
         rc = fail(ECM_E_HIP, "event creation failed");
     for (int i = 0; i < 5 && !rc; ++i)
@@ -1008,6 +1027,42 @@ static void invalidate_sorted(ecm_sim* s) {
     s->pending_dead.clear();
 }
 
+// The path points are appended as agents spawn and replan, in no spatial
+// order.  Every kPathRepackSteps steps the live paths are packed in the last
+// step's Morton order (into path_alt, then copied back so the kernels keep
+// the pool's address): an agent's points then share cache lines with its
+// neighbours' instead of costing one or two lines of their own.  The packed
+// total the device counts is checked against the host's at the next repack.
+static constexpr int kPathRepackSteps = 64;
+
+static int repack_paths(ecm_sim* s) {
+    if (s->repack_pending_check) {
+        // long finished: the stream has run 64 steps since
+        if (s->h_repack[1] != 0u || s->h_repack[0] != (uint32_t)s->repack_expect)
+            return fail(ECM_E_STATE, "path repack packed %u points (expected %llu, skipped %u)", s->h_repack[0],
+                        (unsigned long long)s->repack_expect, s->h_repack[1]);
+        s->repack_pending_check = false;
+    }
+    const size_t live = live_path_points(s);
+    if (live == 0 || live > s->path_pool.n || s->path_pool.n > 0xffffffffu) return 0;
+    if (s->path_alt.n < s->path_pool.n) {
+        if (int rc = settle(s)) return rc;
+        s->path_alt.release();
+        if (int rc = s->path_alt.alloc(s->path_pool.n)) return rc;
+    }
+    const int o = s->cur ^ 1;
+    HIPCHK(hipMemsetAsync(s->repack_ctr.p, 0, 2 * sizeof(uint32_t), s->stream));
+    HIPCHK(launch_path_repack(s->prm.max_agents, s->acc.p, s->sorted_pr[o].p, s->dead_s[o].p, s->path_s[o].p,
+                              s->path_off.p, s->path_pool.p, s->path_alt.p, (uint32_t)s->path_pool.n,
+                              s->repack_ctr.p, s->stream));
+    HIPCHK(hipMemcpyAsync(s->path_pool.p, s->path_alt.p, live * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_repack, s->repack_ctr.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    s->path_used = live;
+    s->repack_expect = live;
+    s->repack_pending_check = true;
+    return 0;
+}
+
 // Before a step that enters from the last step's sorted outputs: agents the
 // host removed are marked destroyed at their last sorted position, and the
 // ids of agents spawned since then go to the device with their count
@@ -1031,6 +1086,12 @@ static int upload_pending(ecm_sim* s) {
         const int n = (int)s->dead_stage.size();
         HIPCHK(hipMemcpyAsync(s->dead_ids.p, s->dead_stage.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
         HIPCHK(launch_mark_dead(s->dead_ids.p, n, s->pos_of.p, s->sorted_pr[o].p, s->dead_s[o].p, s->acc.p, s->stream));
+    }
+    // every live agent is in the last step's sorted outputs unless some were
+    // spawned since
+    if (++s->steps_since_repack >= kPathRepackSteps && s->pending_new.empty()) {
+        if (int rc = repack_paths(s)) return rc;
+        s->steps_since_repack = 0;
     }
     if (!s->pending_new.empty()) {
         s->new_stage.swap(s->pending_new);

@@ -1934,6 +1934,51 @@ __global__ void k_path_resort(const unsigned long long* acc, const float4* sorte
     path_s_old[t] = make_int2(path_off[id], path_cnt[id]);
 }
 
+// The live paths packed into `out` in the last step's Morton order, so that
+// neighbouring agents read neighbouring points: each wavefront takes one
+// contiguous range (one atomic on *ctr, zero at launch) for its lanes'
+// paths in lane order.  Agents destroyed in that step or removed since
+// (dead_s_old) are dropped.  The paths of live agents are disjoint ranges of
+// the pool, so the packed total is at most the pool's length `cap`; a lane
+// whose range would pass it is counted in ctr[1] instead (never expected).
+__global__ void k_path_repack(const unsigned long long* acc, const float4* sorted_pr_old, const uint8_t* dead_s_old,
+                              int2* path_s_old, int* path_off, const float2* pool, float2* out, uint32_t cap,
+                              uint32_t* ctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < (int)(uint32_t)acc[13] && !dead_s_old[t];
+    const int2 pc = live ? path_s_old[t] : make_int2(0, 0);
+    const uint32_t cnt = live ? (uint32_t)pc.y : 0u;
+    // exclusive prefix of cnt over the wavefront, then one atomic for its total
+    const int lane = (int)__lane_id();
+    uint32_t pre = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(pre, d);
+        if (lane >= d) pre += o;
+    }
+    const uint32_t total = __shfl(pre, 63);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(ctr, total);
+    base = __shfl(base, 0) + pre - cnt;
+    if (!live || cnt == 0u) return;
+    if ((uint64_t)base + cnt > cap || pc.x < 0 || (uint64_t)pc.x + cnt > cap) {
+        atomicAdd(ctr + 1, 1u);                        // counted as skipped
+        return;
+    }
+    for (uint32_t q = 0; q < cnt; ++q) out[base + q] = pool[(uint32_t)pc.x + q];
+    path_s_old[t] = make_int2((int)base, pc.y);
+    path_off[__float_as_int(sorted_pr_old[t].w)] = (int)base;
+}
+
+hipError_t launch_path_repack(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
+                              const uint8_t* dead_s_old, int2* path_s_old, int* path_off, const float2* pool,
+                              float2* out, uint32_t cap, uint32_t* ctr, hipStream_t st) {
+    if (n_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_path_repack, dim3(cdiv(n_slots, 256)), dim3(256), 0, st, acc, sorted_pr_old, dead_s_old,
+                       path_s_old, path_off, pool, out, cap, ctr);
+    return hipGetLastError();
+}
+
 hipError_t launch_path_resort(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
                               const int* path_off, const int* path_cnt, int2* path_s_old, hipStream_t st) {
     if (n_slots <= 0) return hipSuccess;

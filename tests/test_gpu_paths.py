@@ -128,3 +128,53 @@ def test_corrupt_snapshots_are_rejected(native_gpu):
     res = compare_step(sim, before, ora)
     assert all(v == 0 for v in res.values()), res
     sim.close()
+
+
+def test_repacked_pool_keeps_each_agents_path(native_gpu):
+    """The pool is repacked in Morton order every 64 steps: over 200 steps
+    (three repacks, agents reaching their goals and being destroyed, a spawn
+    batch in between) every active agent keeps exactly the points it was
+    given, the pool shrinks to the live points, and a step after the last
+    repack equals the oracle's bit for bit.  No planner is attached, so no
+    path changes on its own."""
+    from oracle import oracle_py as O
+    box, rects, obs, ecm, agents = small_scene(12, half=40.0, n_rects=4, n_agents=400)
+    p = Params(max_agents=512, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    ids = [int(i) for i in sim.SpawnAgents(agents) if i >= 0]
+
+    def snapshot_paths():
+        off, cnt, pool = sim.paths()
+        act = sim.state()["active"].astype(bool)
+        return {i: pool[off[i]: off[i] + cnt[i]].copy() for i in np.flatnonzero(act)}, pool.shape[0], int(cnt[act].sum())
+
+    want, _, _ = snapshot_paths()
+    n_destroyed = 0
+    live_before = None
+    for step in range(200):
+        sim.Update()
+        if step == 100:
+            # spawn the agents that reached their goals again, with new ids' paths recorded
+            _, _, _, _, more = small_scene(13, half=40.0, n_rects=4, n_agents=60)
+            for i in sim.SpawnAgents(more):
+                if i >= 0:
+                    off, cnt, pool = sim.paths()
+                    want[int(i)] = pool[off[i]: off[i] + cnt[i]].copy()
+        if step in (63, 64, 127, 128, 191, 192, 199):
+            got, pool_n, live = snapshot_paths()
+            for i, pts in got.items():
+                assert np.array_equal(pts, want[i]), (step, i)
+            n_destroyed = len(want) - len(got)
+            print(f"step {step}: {len(got)} active, pool {pool_n} points, live {live}", flush=True)
+            # the repack before steps 64, 128 and 192 keeps the paths of the
+            # agents active after the step before
+            if step in (64, 128, 192):
+                assert pool_n == live_before, (step, pool_n, live_before)
+            live_before = live
+    assert n_destroyed > 0
+    before = oracle_state(sim)
+    ora = O.step(p, before, obs, ecm)
+    sim.Update()
+    res = compare_step(sim, before, ora)
+    assert all(v == 0 for v in res.values()), res
+    sim.close()
