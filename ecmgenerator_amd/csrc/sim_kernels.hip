@@ -1548,8 +1548,14 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
 // nObst slots, in scan order) and the K agent constraints, RandomizedLP
 // over them (lp2_reg), then integration or the 3D-LP queue.  An agent with
 // more than OB obstacle constraints is parked untouched for k_orca_ovf.
+// At K 10 and OB 4 the pass is capped at 80 VGPRs for 6 waves per SIMD: a
+// small spill (68 B per lane) costs less than the latency the sixth wave
+// hides (ORCA phase 0.45 -> 0.42 ms at c2's dense window; 7 and 8 waves
+// spill 156 and 192 B and are slower).
+constexpr int orca_reg_waves(int K, int OB) { return K <= 10 && OB <= 4 ? 6 : 1; }
+
 template <int K, int OB>
-__global__ __launch_bounds__(64) void k_orca_reg(DevState s) {
+__global__ __launch_bounds__(64, orca_reg_waves(K, OB)) void k_orca_reg(DevState s) {
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * 64 + threadIdx.x;
     if (t >= (int)n_active) return;
