@@ -160,7 +160,7 @@ def main():
     mean_path = mean_path_points(sim)
     # per-kernel HIP-event timing on the handle's stream, separate steps
     sim.set_timing(True)
-    kt = np.zeros(4)
+    kt = np.zeros(5)
     n_kt = max(3, min(args.steps, 10))
     for _ in range(n_kt):
         sim.Update()
@@ -170,9 +170,14 @@ def main():
     wall = t1 - t0
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / wall
-    kt_mean = np.delete(kt / n_kt, 2)          # no proposal phase in the plain step
-    names = ["grid_build", "attraction_knn", "orca_lp_integrate"]
-    dom = int(np.argmax(kt_mean))
+    kt = kt / n_kt
+    # phases (HIP events on the simulator's stream); no proposal phase in the plain step
+    phases = {"grid_build": kt[0], "attraction_knn": kt[1], "orca_lp_integrate": kt[3],
+              "orca_main": kt[4], "orca_queues": kt[3] - kt[4]}
+    # the roofline is quoted on the longest single kernel: the attraction /
+    # k-nearest pass (k_attr_knn) or the main ORCA pass (k_orca_reg)
+    dom_name = "attraction_knn" if phases["attraction_knn"] >= phases["orca_main"] else "orca_main"
+    dom_ms = phases[dom_name]
     n_per_step = updates / args.steps
     # algorithmic HBM bytes per agent-update: the state one update must read
     # and write once (pos 8, vel 8, radius 4, preferred speed 4, path offset
@@ -180,24 +185,24 @@ def main():
     # pos 8, vel 8, attraction 8, active 1), 58 + 8 P.  Neighbour gathers,
     # the grid passes and verification outputs are not counted.
     b_agent = 58.0 + 8.0 * mean_path
-    achieved = b_agent * n_per_step / (kt_mean[dom] * 1e-3) / 1e9
+    achieved = b_agent * n_per_step / (dom_ms * 1e-3) / 1e9
     # each kernel's own minimum state traffic per agent, for comparison
     own_bytes = {
         "grid_build": 1 + 8 + 8 + 4 + 4 + 4 + 16 + 8 + 4,
         "attraction_knn": 16 + 8 + 8 + 8 * mean_path + 8 + 4 + 8 + 8 + 4 * wl.params.k + 4,
-        "orca_lp_integrate": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
+        "orca_main": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
     }
-    traffic, tsrc = _profiled(name, names[dom], "traffic")
-    valu, _ = _profiled(name, names[dom], "valu_busy")
+    traffic, tsrc = _profiled(name, dom_name, "traffic")
+    valu, _ = _profiled(name, dom_name, "valu_busy")
     step_gbs = b_agent * n_per_step / (ms_per_step * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None if traffic is None else int(traffic),
             "traffic_source": tsrc, "valu_busy": valu,
-            "kernel": names[dom], "bytes_per_agent_update": round(b_agent, 2),
+            "kernel": dom_name, "kernel_ms": round(float(dom_ms), 4), "bytes_per_agent_update": round(b_agent, 2),
             "algorithmic_bytes_per_launch": int(b_agent * n_per_step),
             "whole_step": {"achieved": round(step_gbs, 3), "frac": round(step_gbs / HBM_PEAK_GBS, 6)},
             "kernel_own_bytes_per_agent": {k: round(v, 2) for k, v in own_bytes.items()},
-            "kernel_ms": {k: round(float(v), 4) for k, v in zip(names, kt_mean)}}
+            "phase_ms": {k: round(float(v), 4) for k, v in phases.items()}}
 
     graph_on, graph_why = sim.graph_status()
     learned = None
@@ -308,7 +313,7 @@ def learned_leg(sim, wl, snap, wpath, args):
     t1 = time.perf_counter()
     st1 = sim.stats()
     sim.set_timing(True)
-    kt = np.zeros(4)
+    kt = np.zeros(5)
     n_kt = max(3, min(args.steps, 10))
     for _ in range(n_kt):
         sim.Update()
@@ -326,8 +331,9 @@ def learned_leg(sim, wl, snap, wpath, args):
             "rejected_full_2d_lp": tried - acc, "lp3d": st1["lp3d"] - st0["lp3d"],
             "max_dv_vs_plain": max_dv, "max_dp_vs_plain": max_dp, "tolerance_dv": 1e-4,
             "theta": theta, "weights": os.path.relpath(wpath, ROOT),
-            "kernel_ms": {"grid_build": round(float(kt[0]), 4), "attraction_knn": round(float(kt[1]), 4),
-                          "features_mlp": round(float(kt[2]), 4), "orca_verify_integrate": round(float(kt[3]), 4)},
+            "phase_ms": {"grid_build": round(float(kt[0]), 4), "attraction_knn": round(float(kt[1]), 4),
+                         "features_mlp": round(float(kt[2]), 4), "orca_verify_integrate": round(float(kt[3]), 4),
+                         "orca_verify_main": round(float(kt[4]), 4)},
             # network FLOP rate over the whole proposal phase (features + MLP)
             "proposal_tflops": round(85248.0 * (upd / args.steps) / (kt[2] * 1e-3) / 1e12, 2) if kt[2] > 0 else None}
 

@@ -30,7 +30,7 @@ hipError_t launch_unsort(const DevState& s, hipStream_t st);
 hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
                             const unsigned long long* acc, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
-hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st);
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st, hipEvent_t mid);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
 hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, const float* rad, const float* spd,
@@ -230,8 +230,8 @@ struct ecm_sim {
     int replan_threads = 1;
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    float last_ms[4] = {0, 0, 0, 0};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // [5]: after the main ORCA pass
+    float last_ms[5] = {0, 0, 0, 0, 0};
 
     // learned proposal (features + network + verification) and sample log
     ecm_mlp* mlp = nullptr;
@@ -444,7 +444,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipEventCreateWithFlags(&s->ev_stage, hipEventDisableTiming) != hipSuccess)
         rc = fail(ECM_E_HIP, "event creation failed");
-    for (int i = 0; i < 5 && !rc; ++i)
+    for (int i = 0; i < 6 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc) {
         rc |= zero_dev(s->pos.p, N * sizeof(float2), s->stream);
@@ -1537,7 +1537,7 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
                                      s->mlp_theta, s->mlp_mask.p, nullptr, nullptr, s->mlp->n_cu, s->stream));
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
-    HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->n_cu, s->stream));
+    HIPCHK(launch_orca(d, s->maxo, mode, s->tune.orca_obst_slots, s->n_cu, s->stream, s->timing ? s->ev[5] : nullptr));
     if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
     return 0;
 }
@@ -1701,6 +1701,9 @@ static int step_once(ecm_sim* s) {
             HIPCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
             s->last_ms[i] = ms;
         }
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev[3], s->ev[5]));
+        s->last_ms[4] = ms;
     }
     return 0;
 }
@@ -1987,7 +1990,7 @@ extern "C" int ecm_sim_set_timing(ecm_sim* s, int on) {
 }
 extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out, int n) {
     if (!s || !out || n < 0) return fail(ECM_E_ARG, "null argument");
-    for (int i = 0; i < n && i < 4; ++i) out[i] = s->last_ms[i];
+    for (int i = 0; i < n && i < 5; ++i) out[i] = s->last_ms[i];
     return 0;
 }
 

@@ -1829,11 +1829,13 @@ constexpr int kLp3dRegObst = 4;
 constexpr int kLp3dRegBlocks = 8;
 
 template <int K>
-static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st) {
+static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st,
+                          hipEvent_t mid) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
     if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
     else if (oslots <= 2) launch_orca_main<K, 2>(s, mode, g, b, st);
     else launch_orca_main<K, 4>(s, mode, g, b, st);
+    if (mid) (void)hipEventRecord(mid, st);
     const dim3 gr(std::max(1, n_cu) * kLp3dRegBlocks), br(256);
     if (maxo <= 16) {
         hipLaunchKernelGGL((k_orca_ovf<K, 16>), gq, b, 0, st, s);
@@ -1866,11 +1868,12 @@ hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
 }
 
 // mode: 0 plain, 1 plain + sample log, 2 learned proposal (needs the
-// predicted masks of launch_features + the proposal network)
-hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st) {
-    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, oslots, n_cu, st);
-    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, oslots, n_cu, st);
-    else launch_orca_k<16>(s, maxo, mode, oslots, n_cu, st);
+// predicted masks of launch_features + the proposal network).  `mid`, when
+// not null, is recorded between the main pass and the queue passes.
+hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st, hipEvent_t mid) {
+    if (s.k <= 5) launch_orca_k<5>(s, maxo, mode, oslots, n_cu, st, mid);
+    else if (s.k <= 10) launch_orca_k<10>(s, maxo, mode, oslots, n_cu, st, mid);
+    else launch_orca_k<16>(s, maxo, mode, oslots, n_cu, st, mid);
     return hipGetLastError();
 }
 

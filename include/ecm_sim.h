@@ -207,7 +207,9 @@ int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n);
 /* Device time of the phases of the last step in ms (HIP events on the
  * handle's stream): [0] grid build, [1] attraction + k-nearest, [2] learned
  * proposal (features + network; 0 without one), [3] ORCA + LP +
- * integration.  Enable with ecm_sim_set_timing(s, 1); copies min(n, 4). */
+ * integration, [4] of [3], the main ORCA pass alone (the rest is the
+ * passes over its queues: obstacle overflow and the 3D LP).  Enable with
+ * ecm_sim_set_timing(s, 1); copies min(n, 5). */
 int ecm_sim_set_timing(ecm_sim* s, int on);
 int ecm_sim_last_timing(ecm_sim* s, float* out, int n);
 
