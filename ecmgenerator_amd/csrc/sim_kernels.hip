@@ -1681,19 +1681,29 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
     }
 }
 
+// Queues up to this many entries per wavefront of the grid are dealt
+// lane-major in k_lp3d_reg.
+static constexpr uint32_t kLp3dSpreadLanes = 4;
+
 // The 3D-LP queue (agents whose 2D LP failed), then integration: constraints
 // in registers (up to OB obstacle and K agent constraints; entries with more
-// obstacle constraints go to q3d_big for k_lp3d).  Each wavefront takes 64
-// consecutive entries (coalesced slot-major loads); wavefronts are dealt to
-// the blocks round robin so that a short queue still spreads over the chip.
+// obstacle constraints go to q3d_big for k_lp3d).  A long queue: each
+// wavefront takes 64 consecutive entries (coalesced slot-major loads),
+// wavefronts dealt to the blocks round robin.  A short one (at most
+// kLp3dSpreadLanes entries per wavefront of the grid): entry q goes to lane
+// q / nw of wavefront q % nw, so that each wavefront holds a few entries and
+// runs only their iterations of the candidate loop (the loop is
+// wave-uniform: a full wavefront runs the union of its lanes').
 template <int K, int OB>
 __global__ __launch_bounds__(256, 4) void k_lp3d_reg(DevState s) {
     const uint32_t nq = s.stats[0];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wpb = blockDim.x >> 6;
     const uint32_t nw = gridDim.x * wpb;
-    for (uint32_t w = blockIdx.x + gridDim.x * (threadIdx.x >> 6); w * 64u < nq; w += nw) {
-        const uint32_t q = w * 64u + lane;
+    const uint32_t w0 = blockIdx.x + gridDim.x * (threadIdx.x >> 6);
+    const bool spread = nq <= nw * kLp3dSpreadLanes;
+    for (uint32_t w = w0; spread ? w == w0 : w * 64u < nq; w += nw) {
+        const uint32_t q = spread ? lane * nw + w : w * 64u + lane;
         if (q >= nq) continue;
         const int4 info = s.q3d_info[q];
         const int nObst = info.y, na = info.x - info.y;
