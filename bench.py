@@ -175,8 +175,15 @@ def main():
     phases = {"grid_build": kt[0], "attraction_knn": kt[1], "orca_lp_integrate": kt[3],
               "orca_main": kt[4], "orca_queues": kt[3] - kt[4]}
     # the roofline is quoted on the longest single kernel: the attraction /
-    # k-nearest pass (k_attr_knn) or the main ORCA pass (k_orca_reg)
-    dom_name = "attraction_knn" if phases["attraction_knn"] >= phases["orca_main"] else "orca_main"
+    # k-nearest pass (k_attr_knn) or the main ORCA pass (k_orca_reg), or the
+    # kernel that runs both (k_attr_orca: then the main ORCA phase is empty)
+    shape = sim.get_tuning()
+    fused = wl.params.k in (5, 10) and shape["orca_obst_slots"] == 2
+    if fused:
+        phases = {"grid_build": kt[0], "attraction_knn_orca_main": kt[1], "orca_queues": kt[3]}
+        dom_name = "attraction_knn_orca_main"
+    else:
+        dom_name = "attraction_knn" if phases["attraction_knn"] >= phases["orca_main"] else "orca_main"
     dom_ms = phases[dom_name]
     n_per_step = updates / args.steps
     # algorithmic HBM bytes per agent-update: the state one update must read
@@ -192,6 +199,10 @@ def main():
         "attraction_knn": 16 + 8 + 8 + 8 * mean_path + 8 + 4 + 8 + 8 + 4 * wl.params.k + 4,
         "orca_main": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
     }
+    # the kernel running both passes reads the agent's state (16 + 8), its
+    # destroyed flag, preferred velocity, neighbour count and list once
+    own_bytes["attraction_knn_orca_main"] = (own_bytes["attraction_knn"] + own_bytes["orca_main"]
+                                             - (16 + 8 + 1 + 8 + 4 + 4 * wl.params.k))
     traffic, tsrc = _profiled(name, dom_name, "traffic")
     valu, _ = _profiled(name, dom_name, "valu_busy")
     step_gbs = b_agent * n_per_step / (ms_per_step * 1e-3) / 1e9

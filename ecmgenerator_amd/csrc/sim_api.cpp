@@ -29,7 +29,7 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_unsort(const DevState& s, hipStream_t st);
 hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
                             const unsigned long long* acc, hipStream_t st);
-hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st);
+hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st, hipEvent_t mid);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
@@ -1523,10 +1523,12 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
         // into the totals and clears them for this step
         HIPCHK(launch_grid_build(d, s->bsum.p, s->stream));
         if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
-        HIPCHK(launch_attr_knn(d, s->tune.knn_block, s->stream));
-        // replan and destroy events are final once k_attr_knn is done; its
+        // the plain step's main ORCA pass may run inside this kernel (k 5
+        // and 10): its events are published once both are done
+        HIPCHK(launch_attr_knn(d, s->tune.knn_block, mode, s->tune.orca_obst_slots, s->stream));
+        // replan and destroy events are final once this kernel is done; its
         // last block writes them to host memory, and the host handles them
-        // while k_orca runs
+        // while the ORCA kernels (or, fused, the queue passes) run
         return 0;
     }
     if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));

@@ -1555,10 +1555,40 @@ __global__ __launch_bounds__(64) void k_orca(DevState s) {
 constexpr int orca_reg_waves(int K, int OB) { return K <= 10 && OB <= 4 ? 6 : 1; }
 
 template <int K, int OB>
+__device__ __forceinline__ void orca_reg_agent(const DevState& s, int t);
+
+template <int K, int OB>
 __global__ __launch_bounds__(64, orca_reg_waves(K, OB)) void k_orca_reg(DevState s) {
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * 64 + threadIdx.x;
     if (t >= (int)n_active) return;
+    orca_reg_agent<K, OB>(s, t);
+}
+
+// The plain step's attraction, steering and k-nearest pass followed, in the
+// same thread, by the main ORCA pass of the same agent (k 5 and 10, 4
+// obstacle slots): the agent's ORCA reads only step-start state of other
+// agents and its own new neighbour list and preferred velocity, so no
+// other thread's result is waited for.  One kernel instead of two: the
+// neighbour list is read back from the thread's own writes, and the tail of
+// one pass overlaps the other's.  The events are published at its end, so
+// the host's replans overlap only the queue passes.  Capped at 80 VGPRs for
+// 6 waves per SIMD (88 B of spill): 0.682 -> 0.644 ms of kernel time per
+// step at c2's dense window against the two kernels, 0.651 uncapped; but
+// with c2's ~90 replans per step the wall time grows (0.75 -> 0.83 ms).
+template <int K>
+__global__ __launch_bounds__(256, 6) void k_attr_orca(DevState s) {
+    extern __shared__ int knn_lds[];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.acc[15] = 0;   // the grid build has used it
+    attr_knn_agent<K>(s, knn_lds);
+    const uint32_t n_active = s.cell_start[s.n_keys];
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t < (int)n_active) orca_reg_agent<K, 4>(s, t);
+    publish_if_last(s);
+}
+
+template <int K, int OB>
+__device__ __forceinline__ void orca_reg_agent(const DevState& s, int t) {
     const float4 pr = s.sorted_pr[t];
     const float4 sv = s.sorted_v[t];
     const float2 v0 = make_float2(sv.x, sv.y);
@@ -1833,6 +1863,14 @@ static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStr
     else hipLaunchKernelGGL((k_orca_reg<K, OB>), g, b, 0, st, s);
 }
 
+// Whether the plain step's main ORCA pass runs inside the attraction /
+// k-nearest kernel (k_attr_orca): the launch shape's orca_obst_slots 2
+// selects it for k 5 and 10.  It saves kernel time, but the kernel then
+// publishes the step's replan events only once the main ORCA pass is done
+// too, so the host's replanning overlaps less of the step; the autotuner
+// weighs the two on the live workload.
+bool orca_fused(int k, int mode, int oslots) { return mode == kPlain && (k == 5 || k == 10) && oslots == 2; }
+
 // Obstacle constraints held in registers by k_lp3d_reg; its persistent
 // grid is kLp3dRegBlocks workgroups of 256 per CU.
 constexpr int kLp3dRegObst = 4;
@@ -1842,7 +1880,9 @@ template <int K>
 static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st,
                           hipEvent_t mid) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
-    if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
+    if (orca_fused(s.k, mode, oslots)) {
+        // the main pass ran inside k_attr_orca
+    } else if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
     else if (oslots <= 2) launch_orca_main<K, 2>(s, mode, g, b, st);
     else launch_orca_main<K, 4>(s, mode, g, b, st);
     if (mid) (void)hipEventRecord(mid, st);
@@ -1862,10 +1902,15 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int
     }
 }
 
-hipError_t launch_attr_knn(const DevState& s, int block, hipStream_t st) {
+hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st) {
     if (block != 64 && block != 128) block = 256;
     dim3 g(round8(cdiv(s.n_slots, block))), b(block);
     const size_t lds = (size_t)(kKnnBuf + 2 * kKnnRanges) * block * sizeof(int);
+    if (orca_fused(s.k, mode, oslots)) {
+        if (s.k == 5) hipLaunchKernelGGL((k_attr_orca<5>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_attr_orca<10>), g, b, lds, st, s);
+        return hipGetLastError();
+    }
     switch (s.k) {
 #define ECM_KNN_CASE(k) case k: hipLaunchKernelGGL((k_attr_knn<k>), g, b, lds, st, s); break;
         ECM_KNN_CASE(1) ECM_KNN_CASE(2) ECM_KNN_CASE(3) ECM_KNN_CASE(4) ECM_KNN_CASE(5) ECM_KNN_CASE(6)

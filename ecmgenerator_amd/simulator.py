@@ -316,6 +316,13 @@ class Simulator:
         t = N.SimTuning(float(cell_scale), int(knn_block), int(orca_obst_slots), 0.0)
         N.check(self._L.ecm_sim_set_tuning(self._h, C.byref(t)))
 
+    def get_tuning(self) -> dict:
+        """The launch shape in use."""
+        t = N.SimTuning()
+        N.check(self._L.ecm_sim_get_tuning(self._h, C.byref(t)))
+        return {"cell_scale": round(t.cell_scale, 3), "knn_block": t.knn_block, "orca_obst_slots": t.orca_obst_slots,
+                "best_ms": round(t.best_ms, 4)}
+
     def autotune(self, steps: int = 3) -> dict:
         """Pick the launch shape by timing trials on this workload (advances
         the simulation); returns the chosen shape and its step time."""

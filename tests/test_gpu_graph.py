@@ -86,3 +86,29 @@ def test_autotune_changes_speed_not_results():
         same(a, b)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("k", [5, 10])
+def test_orca_pass_shapes_agree(k):
+    """The main ORCA pass as its own kernel with 4 obstacle slots (shape 1),
+    inside the attraction / k-nearest kernel (shape 2, k 5 and 10) and with
+    8 obstacle slots (shape 4) gives the same steps bit for bit, with step
+    graphs and eager launches, replans included."""
+    sc, agents, _ = generate(0, seed=7)
+    sims = []
+    for oslots, graph in ((1, False), (2, True), (2, False), (4, True)):
+        p = Params(max_agents=agents.n + 64, keep_orca_velocity=True, k=k)
+        sim = Simulator(sc.ecm(), sc.obstacles(), sc.box, p, planner=sc)
+        sim.SpawnAgents(agents)
+        sim.set_tuning(orca_obst_slots=oslots)
+        sim.set_graph(graph)
+        sims.append(sim)
+    for step in range(40):
+        for sim in sims:
+            sim.Update()
+        if step % 8 == 7:
+            for sim in sims[1:]:
+                same(sims[0], sim)
+    print(f"k {k}: replans {sims[0].stats()['replans']}, 3D LP {sims[0].stats()['lp3d']}", flush=True)
+    for sim in sims:
+        sim.close()
