@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--theta", type=float, default=None, help="proposal logit threshold (default: from the weights file)")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of replaying step graphs")
-    ap.add_argument("--autotune", type=int, default=3,
+    ap.add_argument("--autotune", type=int, default=5,
                     help="steps per autotuner trial during warmup (0: default launch shape)")
     ap.add_argument("--prerun", type=int, default=None,
                     help="untimed steps before warmup (default: the workload's own, e.g. until the c2 jam forms)")

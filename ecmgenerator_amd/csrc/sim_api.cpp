@@ -1783,26 +1783,39 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
         if (!s->envs[e]->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before autotune (environment %zu)", e);
     (void)hipSetDevice(s->dev);
     if (int rc_ = settle(s)) return rc_;
+    // A trial: one untimed step (it recaptures the step graphs), then the
+    // median of `steps` step times, each from one step's return to the next
+    // (a step returns once its events are read and its replans uploaded, so
+    // the interval covers the device work the host waits for and the host's
+    // own).  The median drops the steps disturbed by host noise.
     auto trial = [&](const ecm_sim_tuning& t, float* ms) -> int {
         apply_tuning(s, t);
         int rc = step_once(s);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(s->stream));
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; i < steps; ++i)
+        std::vector<float> dt;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps; ++i) {
             if ((rc = step_once(s)) != 0) return rc;
-        HIPCHK(hipStreamSynchronize(s->stream));
-        *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+            if (i + 1 == steps) HIPCHK(hipStreamSynchronize(s->stream));
+            const auto t1 = std::chrono::steady_clock::now();
+            dt.push_back(std::chrono::duration<float, std::milli>(t1 - t0).count());
+            t0 = t1;
+        }
+        std::sort(dt.begin(), dt.end());
+        *ms = dt[dt.size() / 2];
         return 0;
     };
     ecm_sim_tuning best = s->tune;
     float best_ms = 0.0f;
-    int rc = trial(best, &best_ms);
-    if (rc) return rc;
+    int rc = 0;
     const float scales[] = {0.6f, 0.8f, 1.0f, 1.25f, 1.6f, 2.0f};
     const int blocks[] = {64, 128, 256};
     const int slots[] = {1, 2, 4};
     for (int dim = 0; dim < 3; ++dim) {
+        // the incumbent is timed again at the start of each coordinate: the
+        // workload moves on while the trials run
+        if ((rc = trial(best, &best_ms)) != 0) return rc;
         const int nopt = dim == 0 ? 6 : 3;
         for (int o = 0; o < nopt; ++o) {
             ecm_sim_tuning t = best;
