@@ -129,27 +129,39 @@ ECM_HD bool lp_dir_reg(const V2 (&on)[OB], const V2 (&op)[OB], int o_lo, const V
 }
 
 // RandomizedLP3D from reference index `failed` on, out holding the 2D LP's
-// result at the failure.  The loop over the candidate constraint ci runs
-// over slots, the same in every lane of a wavefront, so the number of
-// bisectors and the inner LP's length are wave-uniform; a lane takes part
-// from its failing slot on.  fetch(c) returns constraint c as (nx, ny, px,
-// py): re-reading it is cheaper than selecting a register by the loop
-// position, which would make the compiler index the arrays in memory.
+// result at the failure.  Each lane walks its own candidates ci from its
+// failing slot on, skipping those whose penetration is within maxPen (a
+// load and a determinant each), and the wavefront runs the bisectors and
+// the inner LP once per round for every lane that found its next
+// candidate.  A lane needs the inner LP for only one or two candidates, but
+// which ones differs from lane to lane: a loop over candidates shared by
+// the wavefront would run the union of its lanes' candidates.  The
+// bisectors of a round are built for j < jv of each lane (unrolled, no
+// register selected by position); fetch(c) returns constraint c as (nx, ny,
+// px, py).
 template <int OB, int K, typename Fetch>
 ECM_HD void lp3d_reg(const V2 (&on)[OB], const V2 (&op)[OB], const V2 (&an)[K], const V2 (&ap)[K], int nObst, int na,
                      int failed, float maxSpeed, Fetch&& fetch, V2& out) {
     const int o_lo = OB - nObst;
-    const int sfail = index_to_slot<OB>(failed, nObst);
+    const int end = OB + na;                       // the list's slots are [o_lo, end)
+    int v = index_to_slot<OB>(failed, nObst);
     float maxPen = 0.0f;
-#pragma unroll 1
-    for (int v = 0; v < OB + K; ++v) {
+    while (true) {
+        // this lane's next candidate whose penetration exceeds maxPen
+        V2 cn = mk(0.0f, 0.0f), cp = mk(0.0f, 0.0f);
+        bool found = false;
+        for (; v < end; ++v) {
+            const float4 c4 = fetch(slot_to_index<OB>(v, nObst));
+            cn = mk(c4.x, c4.y);
+            cp = mk(c4.z, c4.w);
+            if (!(det(right(cn), cp - out) <= maxPen)) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
         const int jv = v - OB;
-        const bool in_list = v < OB ? (v >= o_lo) : (jv < na);
-        if (!(v >= sfail && in_list)) continue;
-        const float4 c4 = fetch(slot_to_index<OB>(v, nObst));
-        const V2 cn = mk(c4.x, c4.y), cp = mk(c4.z, c4.w);
         const V2 dir = right(cn);
-        if (det(dir, cp - out) <= maxPen) continue;
         // bisectors with the agent constraints before ci (ORCA.cpp:614-646)
         V2 pn[K], pp[K];
         uint32_t pval = 0u;
@@ -176,6 +188,7 @@ ECM_HD void lp3d_reg(const V2 (&on)[OB], const V2 (&op)[OB], const V2 (&an)[K], 
         const V2 temp = out;
         if (!lp_dir_reg<OB, K>(on, op, o_lo, pn, pp, pval, cn, maxSpeed, out)) out = temp;
         maxPen = det(dir, cp - out);
+        ++v;
     }
 }
 

@@ -1783,27 +1783,23 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
         if (!s->envs[e]->ecm_set) return fail(ECM_E_STATE, "set_ecm must be called before autotune (environment %zu)", e);
     (void)hipSetDevice(s->dev);
     if (int rc_ = settle(s)) return rc_;
-    // A trial: one untimed step (it recaptures the step graphs), then the
-    // median of `steps` step times, each from one step's return to the next
-    // (a step returns once its events are read and its replans uploaded, so
-    // the interval covers the device work the host waits for and the host's
-    // own).  The median drops the steps disturbed by host noise.
-    auto trial = [&](const ecm_sim_tuning& t, float* ms) -> int {
+    // One sample of a shape: two untimed steps after switching to it (they
+    // recapture the step graphs of both buffer parities), then one step
+    // timed from an idle stream to an idle stream (the device work and the
+    // host's part: events, replans).  The options of a coordinate are
+    // sampled round robin, `steps` rounds, and compared by their median: the
+    // crowd moves on while the search runs, and round robin spreads that
+    // drift over every option.
+    auto sample = [&](const ecm_sim_tuning& t, float* ms) -> int {
         apply_tuning(s, t);
-        int rc = step_once(s);
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(s->stream));
-        std::vector<float> dt;
-        auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; i < steps; ++i) {
+        int rc = 0;
+        for (int w = 0; w < 2; ++w)
             if ((rc = step_once(s)) != 0) return rc;
-            if (i + 1 == steps) HIPCHK(hipStreamSynchronize(s->stream));
-            const auto t1 = std::chrono::steady_clock::now();
-            dt.push_back(std::chrono::duration<float, std::milli>(t1 - t0).count());
-            t0 = t1;
-        }
-        std::sort(dt.begin(), dt.end());
-        *ms = dt[dt.size() / 2];
+        HIPCHK(hipStreamSynchronize(s->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = step_once(s)) != 0) return rc;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return 0;
     };
     ecm_sim_tuning best = s->tune;
@@ -1813,19 +1809,26 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
     const int blocks[] = {64, 128, 256};
     const int slots[] = {1, 2, 4};
     for (int dim = 0; dim < 3; ++dim) {
-        // the incumbent is timed again at the start of each coordinate: the
-        // workload moves on while the trials run
-        if ((rc = trial(best, &best_ms)) != 0) return rc;
+        std::vector<ecm_sim_tuning> opts{best};
         const int nopt = dim == 0 ? 6 : 3;
         for (int o = 0; o < nopt; ++o) {
             ecm_sim_tuning t = best;
             if (dim == 0) t.cell_scale = scales[o];
             else if (dim == 1) t.knn_block = blocks[o];
             else t.orca_obst_slots = slots[o];
-            if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;
-            float ms = 0.0f;
-            if ((rc = trial(t, &ms)) != 0) return rc;
-            if (ms < best_ms) { best_ms = ms; best = t; }
+            if (std::memcmp(&t, &best, sizeof(t)) != 0) opts.push_back(t);
+        }
+        std::vector<std::vector<float>> ms(opts.size());
+        for (int r = 0; r < steps; ++r)
+            for (size_t o = 0; o < opts.size(); ++o) {
+                float m = 0.0f;
+                if ((rc = sample(opts[o], &m)) != 0) return rc;
+                ms[o].push_back(m);
+            }
+        for (size_t o = 0; o < opts.size(); ++o) {
+            std::sort(ms[o].begin(), ms[o].end());
+            const float med = ms[o][ms[o].size() / 2];
+            if (o == 0 || med < best_ms) { best_ms = med; best = opts[o]; }
         }
     }
     apply_tuning(s, best);

@@ -130,9 +130,11 @@ typedef struct ecm_sim_tuning {
 } ecm_sim_tuning;
 int ecm_sim_get_tuning(ecm_sim* s, ecm_sim_tuning* out);
 int ecm_sim_set_tuning(ecm_sim* s, const ecm_sim_tuning* t);
-/* Coordinate search of the launch shape on the current workload, `steps`
- * timed steps per trial (plus one untimed); the simulation advances
- * through every trial step.  The best shape is kept and returned. */
+/* Coordinate search of the launch shape on the current workload: the
+ * options of each coordinate are sampled round robin, `steps` rounds of one
+ * timed step each (after two untimed ones), and compared by their median;
+ * the simulation advances through every sampled step.  The best shape is
+ * kept and returned. */
 int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out);
 
 /* Batched multi-scenario stepping.  A handle holds one or more
