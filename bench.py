@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of replaying step graphs")
     ap.add_argument("--autotune", type=int, default=5,
                     help="steps per autotuner trial during warmup (0: default launch shape)")
+    ap.add_argument("--launch-shape", default=None,
+                    help="CELL_SCALE,KNN_BLOCK,ORCA_OBST_SLOTS to use instead of autotuning (with --autotune 0)")
     ap.add_argument("--prerun", type=int, default=None,
                     help="untimed steps before warmup (default: the workload's own, e.g. until the c2 jam forms)")
     return ap.parse_args()
@@ -143,6 +145,10 @@ def main():
     if args.autotune > 0:
         tuning = sim.autotune(args.autotune)
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
+    elif args.launch_shape:
+        cs, kb, os_ = args.launch_shape.split(",")
+        sim.set_tuning(cell_scale=float(cs), knn_block=int(kb), orca_obst_slots=int(os_))
+        tuning = sim.get_tuning()
     # both legs time the same steps: the state just loaded, one untimed step
     # (buffer re-creation after the load), then K timed steps
     first_ms = _first_step_after_load(sim, snap)
@@ -160,7 +166,7 @@ def main():
     mean_path = mean_path_points(sim)
     # per-kernel HIP-event timing on the handle's stream, separate steps
     sim.set_timing(True)
-    kt = np.zeros(5)
+    kt = np.zeros(6)
     n_kt = max(3, min(args.steps, 10))
     for _ in range(n_kt):
         sim.Update()
@@ -171,19 +177,20 @@ def main():
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / wall
     kt = kt / n_kt
-    # phases (HIP events on the simulator's stream); no proposal phase in the plain step
-    phases = {"grid_build": kt[0], "attraction_knn": kt[1], "orca_lp_integrate": kt[3],
-              "orca_main": kt[4], "orca_queues": kt[3] - kt[4]}
-    # the roofline is quoted on the longest single kernel: the attraction /
-    # k-nearest pass (k_attr_knn) or the main ORCA pass (k_orca_reg), or the
-    # kernel that runs both (k_attr_orca: then the main ORCA phase is empty)
+    # phases (HIP events on the simulator's stream; no proposal phase in the
+    # plain step), one kernel each except the grid build and the queue passes
     shape = sim.get_tuning()
     fused = wl.params.k in (5, 10) and shape["orca_obst_slots"] == 2
-    if fused:
-        phases = {"grid_build": kt[0], "attraction_knn_orca_main": kt[1], "orca_queues": kt[3]}
-        dom_name = "attraction_knn_orca_main"
+    phases = {"grid_build": kt[0], "attraction": kt[5]}
+    if fused:           # k_knn_orca: the k-nearest pass and the main ORCA pass in one kernel
+        phases["knn_orca_main"] = kt[1] - kt[5]
     else:
-        dom_name = "attraction_knn" if phases["attraction_knn"] >= phases["orca_main"] else "orca_main"
+        phases["knn"] = kt[1] - kt[5]
+        phases["orca_main"] = kt[4]
+    phases["orca_queues"] = kt[3] - kt[4]
+    # the roofline is quoted on the longest single kernel
+    singles = [k for k in phases if k not in ("grid_build", "orca_queues")]
+    dom_name = max(singles, key=lambda k: phases[k])
     dom_ms = phases[dom_name]
     n_per_step = updates / args.steps
     # algorithmic HBM bytes per agent-update: the state one update must read
@@ -194,15 +201,20 @@ def main():
     b_agent = 58.0 + 8.0 * mean_path
     achieved = b_agent * n_per_step / (dom_ms * 1e-3) / 1e9
     # each kernel's own minimum state traffic per agent, for comparison
+    k = wl.params.k
     own_bytes = {
         "grid_build": 1 + 8 + 8 + 4 + 4 + 4 + 16 + 8 + 4,
-        "attraction_knn": 16 + 8 + 8 + 8 * mean_path + 8 + 4 + 8 + 8 + 4 * wl.params.k + 4,
-        "orca_main": 16 + 8 + 1 + 4 + 8 + 4 + 4 * wl.params.k + 8 + 8,
+        # state 16 + 8, path offset and count 8, path points 8 P, attraction
+        # point read and written 16, preferred velocity 8, destroyed flag 1
+        "attraction": 16 + 8 + 8 + 8 * mean_path + 16 + 8 + 1,
+        # state 16, seeds (previous list 4 k + count 4 + map 4), list and count written 4 k + 4
+        "knn": 16 + 4 * k + 4 + 4 + 4 * k + 4,
+        # state 16 + 8, destroyed flag, radius 4, preferred velocity 8, list
+        # and count 4 k + 4, position and velocity written 8 + 8
+        "orca_main": 16 + 8 + 1 + 4 + 8 + 4 + 4 * k + 8 + 8,
     }
-    # the kernel running both passes reads the agent's state (16 + 8), its
-    # destroyed flag, preferred velocity, neighbour count and list once
-    own_bytes["attraction_knn_orca_main"] = (own_bytes["attraction_knn"] + own_bytes["orca_main"]
-                                             - (16 + 8 + 1 + 8 + 4 + 4 * wl.params.k))
+    # the kernel running both reads the state, the list and its count once
+    own_bytes["knn_orca_main"] = own_bytes["knn"] + own_bytes["orca_main"] - (16 + 4 * k + 4)
     traffic, tsrc = _profiled(name, dom_name, "traffic")
     valu, _ = _profiled(name, dom_name, "valu_busy")
     step_gbs = b_agent * n_per_step / (ms_per_step * 1e-3) / 1e9
@@ -324,7 +336,7 @@ def learned_leg(sim, wl, snap, wpath, args):
     t1 = time.perf_counter()
     st1 = sim.stats()
     sim.set_timing(True)
-    kt = np.zeros(5)
+    kt = np.zeros(6)
     n_kt = max(3, min(args.steps, 10))
     for _ in range(n_kt):
         sim.Update()
@@ -342,7 +354,8 @@ def learned_leg(sim, wl, snap, wpath, args):
             "rejected_full_2d_lp": tried - acc, "lp3d": st1["lp3d"] - st0["lp3d"],
             "max_dv_vs_plain": max_dv, "max_dp_vs_plain": max_dp, "tolerance_dv": 1e-4,
             "theta": theta, "weights": os.path.relpath(wpath, ROOT),
-            "phase_ms": {"grid_build": round(float(kt[0]), 4), "attraction_knn": round(float(kt[1]), 4),
+            "phase_ms": {"grid_build": round(float(kt[0]), 4), "attraction": round(float(kt[5]), 4),
+                         "knn": round(float(kt[1] - kt[5]), 4),
                          "features_mlp": round(float(kt[2]), 4), "orca_verify_integrate": round(float(kt[3]), 4),
                          "orca_verify_main": round(float(kt[4]), 4)},
             # network FLOP rate over the whole proposal phase (features + MLP)

@@ -29,7 +29,7 @@ hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_unsort(const DevState& s, hipStream_t st);
 hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
                             const unsigned long long* acc, hipStream_t st);
-hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st);
+hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st, hipEvent_t mid);
 hipError_t launch_orca(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st, hipEvent_t mid);
 hipError_t launch_features(const DevState& s, hipStream_t st);
 hipError_t launch_export_nbrs(const DevState& s, int* out, int* out_cnt, hipStream_t st);
@@ -183,7 +183,7 @@ struct ecm_sim {
     DBuf<unsigned long long> acc;            // [16] totals (folded in by k_grid_count); [12] steps run
     // host-mapped, 1 + 8 + 4 * kEventPeek words: [0] the number of the last
     // step that published, then its counters and first events (written by
-    // the last block of k_attr_knn)
+    // the last block of k_attr)
     uint32_t* h_xfer = nullptr;
     uint32_t* h_xfer_dev = nullptr;          // the same memory as the kernels see it
     uint32_t step_seq = 0;                   // steps launched (matches acc[12])
@@ -230,8 +230,9 @@ struct ecm_sim {
     int replan_threads = 1;
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // [5]: after the main ORCA pass
-    float last_ms[5] = {0, 0, 0, 0, 0};
+    // [5]: after the main ORCA pass, [6]: after the attraction pass
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float last_ms[6] = {0, 0, 0, 0, 0, 0};
 
     // learned proposal (features + network + verification) and sample log
     ecm_mlp* mlp = nullptr;
@@ -444,7 +445,7 @@ extern "C" int ecm_sim_create(const ecm_sim_params* p, float x0, float y0, float
         rc = fail(ECM_E_HIP, "hipHostMalloc failed");
     if (!rc && hipEventCreateWithFlags(&s->ev_stage, hipEventDisableTiming) != hipSuccess)
         rc = fail(ECM_E_HIP, "event creation failed");
-    for (int i = 0; i < 6 && !rc; ++i)
+    for (int i = 0; i < 7 && !rc; ++i)
         if (hipEventCreate(&s->ev[i]) != hipSuccess) rc = fail(ECM_E_HIP, "event creation failed");
     if (!rc) {
         rc |= zero_dev(s->pos.p, N * sizeof(float2), s->stream);
@@ -1525,7 +1526,8 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
         if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
         // the plain step's main ORCA pass may run inside this kernel (k 5
         // and 10): its events are published once both are done
-        HIPCHK(launch_attr_knn(d, s->tune.knn_block, mode, s->tune.orca_obst_slots, s->stream));
+        HIPCHK(launch_attr_knn(d, s->tune.knn_block, mode, s->tune.orca_obst_slots, s->stream,
+                               s->timing ? s->ev[6] : nullptr));
         // replan and destroy events are final once this kernel is done; its
         // last block writes them to host memory, and the host handles them
         // while the ORCA kernels (or, fused, the queue passes) run
@@ -1544,7 +1546,7 @@ static int enqueue_device_half(ecm_sim* s, const DevState& d, int mode, int half
     return 0;
 }
 
-// Waits until step `seq` has published its events (k_attr_knn writes the
+// Waits until step `seq` has published its events (k_attr writes the
 // step number last, after a system-scope fence).  The stream is polled
 // now and then so that a step that ends without publishing is an error,
 // not a hang.
@@ -1706,6 +1708,8 @@ static int step_once(ecm_sim* s) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, s->ev[3], s->ev[5]));
         s->last_ms[4] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev[1], s->ev[6]));
+        s->last_ms[5] = ms;
     }
     return 0;
 }
@@ -2008,7 +2012,7 @@ extern "C" int ecm_sim_set_timing(ecm_sim* s, int on) {
 }
 extern "C" int ecm_sim_last_timing(ecm_sim* s, float* out, int n) {
     if (!s || !out || n < 0) return fail(ECM_E_ARG, "null argument");
-    for (int i = 0; i < n && i < 5; ++i) out[i] = s->last_ms[i];
+    for (int i = 0; i < n && i < 6; ++i) out[i] = s->last_ms[i];
     return 0;
 }
 

@@ -87,7 +87,7 @@ struct DevState {
     const int2* path_s_old;        // its path (offset, count) per agent
 
     // ---- this step's state by sorted position ----
-    float2* attr_s;                // attraction point (carried in, updated by k_attr_knn)
+    float2* attr_s;                // attraction point (carried in, updated by k_attr)
     float2* pref_vel_s;            // preferred velocity
     uint8_t* dead_s;               // destroyed this step
     float2* pos_s;                 // end-of-step position
@@ -118,7 +118,7 @@ struct DevState {
     int* prev_of;             // [n_slots] this step's sorted position -> previous, -1: none
     int* pos_of;              // [n_slots] agent id -> its sorted position in the last step (k_unsort)
     const int* new_ids;       // agents spawned since the last step, appended to a sorted-entry build
-                              // (their count in acc[15], cleared by k_attr_knn)
+                              // (their count in acc[15], cleared by k_attr)
 
     // ---- events ----
     // One list per step, read back by the host after the neighbour pass:
@@ -127,7 +127,7 @@ struct DevState {
     int4* events;         uint32_t* events_n;
     uint32_t* stats;      // per step: [0] 3D-LP queue length, [1] obstacle-overflow queue
                           // length, [2] locate fallbacks to the exact-y list, [3] blocks of
-                          // k_attr_knn done, [4] 3D-LP entries with more obstacle constraints
+                          // k_attr done, [4] 3D-LP entries with more obstacle constraints
                           // than the register tile holds, [5] unused
     uint32_t* wstats;     // per step, [64][4] slots: learned proposals tried, accepted,
                           // constraints proposed, unused
@@ -137,7 +137,7 @@ struct DevState {
     // step that published, then its counters and first events
     uint32_t* host_xfer;
     uint32_t event_peek;
-    uint32_t* blocks_done;     // blocks of k_attr_knn finished this step
+    uint32_t* blocks_done;     // blocks of k_attr finished this step
     int* ovf_agent;         // overflow queue: sorted positions
     // 3D-LP queue: agents whose 2D LP failed, with their constraints
     int* q3d_agent;         // sorted position

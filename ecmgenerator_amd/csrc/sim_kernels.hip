@@ -4,15 +4,18 @@
 // Step pipeline (all on the handle's stream, no host round trip):
 //   1. k_grid_count / scan / k_grid_scatter: counting sort of the active
 //      agents by Morton cell (replaces KDTree::Construct, KDTree.cpp:22-83).
-//   2. k_attr_knn<K>: goal checks + IRM attraction point on the ECM
+//   2. k_attr: goal checks + IRM attraction point on the ECM
 //      (Simulator.cpp:538-590, IRMPathFollower.cpp:14-115, ECM.cpp:20-96,
-//      ECMCellCollection.cpp:57-90), steering (Simulator.cpp:638-657) and the
-//      exact k-nearest query (replaces KDTree::KNearestAgents, KDTree.cpp:85-202).
-//   3. k_orca<K>: obstacle query (Simulator.cpp:259-292), ORCA half-planes
-//      (ORCA.cpp:60-424) staged in LDS, 2D LP (ORCA.cpp:428-587), velocity
+//      ECMCellCollection.cpp:57-90) and steering (Simulator.cpp:638-657); its
+//      last block publishes the step's replan and destroy events.
+//   3. k_knn<K>: the exact k-nearest query (replaces KDTree::KNearestAgents,
+//      KDTree.cpp:85-202).
+//   4. k_orca_reg<K> (plain step; or fused into k_knn_orca) / k_orca<K>
+//      (learned and logging steps): obstacle query (Simulator.cpp:259-292),
+//      ORCA half-planes (ORCA.cpp:60-424), 2D LP (ORCA.cpp:428-587), velocity
 //      and position integration (Simulator.cpp:592-635, 676-677) fused.
-//      Two compacted queues take the rare cases: k_orca_ovf (more obstacle
-//      constraints than the main tile holds) and k_lp3d (2D LP infeasible,
+//      Compacted queues take the rest: k_orca_ovf (more obstacle constraints
+//      than the main pass holds), k_lp3d_reg and k_lp3d (2D LP infeasible,
 //      RandomizedLP3D, ORCA.cpp:592-669).
 // One agent per lane; lanes are ordered along the Morton curve so a
 // wavefront covers a compact patch of the world.
@@ -291,7 +294,7 @@ __global__ void k_mark_dead(const int* ids, int n, const int* pos_of, const floa
 
 // Brings the slot-indexed arrays up to date from the last step's sorted
 // outputs (the host asks before it reads or changes agents).  Agents
-// destroyed during a step had their slots written then (k_attr_knn).
+// destroyed during a step had their slots written then (k_attr).
 __global__ void k_unsort(DevState s) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int)(uint32_t)s.acc[13]) return;         // the last step's active count
@@ -779,14 +782,14 @@ static constexpr int kKnnBuf = 16;
 static constexpr float kKnnWalkCells = 6.0f;
 static constexpr int kKnnRanges = 4;
 
-// One sorted agent: attraction point, steering, k nearest.  Returns whether
-// it appended an event (a replan or a destroy), after fencing it device-wide.
-template <int K>   // K == k: the k-th neighbour is the last register of the list
-__device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) {
+// One sorted agent's goal check, attraction point and steering (with the
+// destroy or replan request they may make).  Returns whether it appended an
+// event, after fencing it device-wide.
+__device__ __forceinline__ bool attr_agent(const DevState& s) {
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= (int)n_active) return false;
-    float4 pr = s.sorted_pr[t];
+    const float4 pr = s.sorted_pr[t];
     const float4 sv = s.sorted_v[t];
     const int2 pc = s.path_s[t];
     const int id = __float_as_int(pr.w);
@@ -835,7 +838,6 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
         s.active[id] = 0;                                   // DestroyAgent
         uint32_t q = atomicAdd(s.events_n, 1u);
         s.events[q] = make_int4(id, 0, 0, -1);
-        s.nbr_cnt[t] = 0;
         __threadfence();
         return true;
     }
@@ -843,8 +845,22 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     const float speed = sv.z;
     V2 des = normalize(attr - pos) * speed;
     s.pref_vel_s[t] = make_float2(des.x, des.y);
+    if (event) __threadfence();
+    return event;
+}
 
-    // k nearest (replaces KDTree::KNearestAgents)
+// The k nearest of sorted agent t (replaces KDTree::KNearestAgents); an
+// agent destroyed this step gets an empty list.
+template <int K>   // K == k: the k-th neighbour is the last register of the list
+__device__ __forceinline__ void knn_agent(const DevState& s, int* knn_lds, int t) {
+    if (s.dead_s[t]) {
+        s.nbr_cnt[t] = 0;
+        return;
+    }
+    const float4 pr = s.sorted_pr[t];
+    const float4 sv = s.sorted_v[t];
+    const V2 pos = mk(pr.x, pr.y);
+    const EnvDesc& E = env_of_sorted(s, sv);
     int slot[K];
     float sd[K];
     const float bound = knn_seed_bound<K>(s, t, pos, sd, slot);
@@ -861,8 +877,6 @@ __device__ __forceinline__ bool attr_knn_agent(const DevState& s, int* knn_lds) 
     for (int i = 0; i < K; ++i)
         if (i < cnt) s.nbr[(size_t)i * s.n_slots + t] = slot[i];
     s.nbr_cnt[t] = cnt;
-    if (event) __threadfence();
-    return event;
 }
 
 // The block that finishes last copies the step's counters and its first
@@ -894,12 +908,23 @@ __device__ __forceinline__ void publish_if_last(const DevState& s) {
     }
 }
 
-template <int K>
-__global__ __launch_bounds__(256) void k_attr_knn(DevState s) {
-    extern __shared__ int knn_lds[];
+// Goal checks, attraction points and steering of every agent; the block
+// that finishes last publishes the step's events, so the host plans the
+// replans while the neighbour search and the ORCA passes run.  No LDS: the
+// IRM's dependent loads are hidden by as many waves as the registers allow.
+__global__ __launch_bounds__(256) void k_attr(DevState s) {
     if (blockIdx.x == 0 && threadIdx.x == 0) s.acc[15] = 0;   // the grid build has used it
-    attr_knn_agent<K>(s, knn_lds);
+    attr_agent(s);
     publish_if_last(s);
+}
+
+// The k nearest of every agent (kKnnBuf + 2 kKnnRanges ints of LDS per lane).
+template <int K>
+__global__ __launch_bounds__(256) void k_knn(DevState s) {
+    extern __shared__ int knn_lds[];
+    const uint32_t n_active = s.cell_start[s.n_keys];
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t < (int)n_active) knn_agent<K>(s, knn_lds, t);
 }
 
 // ------------------------------------------------------------------- ORCA
@@ -1565,26 +1590,21 @@ __global__ __launch_bounds__(64, orca_reg_waves(K, OB)) void k_orca_reg(DevState
     orca_reg_agent<K, OB>(s, t);
 }
 
-// The plain step's attraction, steering and k-nearest pass followed, in the
-// same thread, by the main ORCA pass of the same agent (k 5 and 10, 4
-// obstacle slots): the agent's ORCA reads only step-start state of other
-// agents and its own new neighbour list and preferred velocity, so no
-// other thread's result is waited for.  One kernel instead of two: the
-// neighbour list is read back from the thread's own writes, and the tail of
-// one pass overlaps the other's.  The events are published at its end, so
-// the host's replans overlap only the queue passes.  Capped at 80 VGPRs for
-// 6 waves per SIMD (88 B of spill): 0.682 -> 0.644 ms of kernel time per
-// step at c2's dense window against the two kernels, 0.651 uncapped; but
-// with c2's ~90 replans per step the wall time grows (0.75 -> 0.83 ms).
+// The plain step's k-nearest pass followed, in the same thread, by the main
+// ORCA pass of the same agent (k 5 and 10, 4 obstacle slots): the agent's
+// ORCA reads only step-start state of other agents and its own new
+// neighbour list and preferred velocity, so no other thread's result is
+// waited for.  One kernel instead of two: the neighbour list is read back
+// from the thread's own writes, and the tail of one pass overlaps the
+// other's.  Capped at 80 VGPRs for 6 waves per SIMD.
 template <int K>
-__global__ __launch_bounds__(256, 6) void k_attr_orca(DevState s) {
+__global__ __launch_bounds__(256, 6) void k_knn_orca(DevState s) {
     extern __shared__ int knn_lds[];
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.acc[15] = 0;   // the grid build has used it
-    attr_knn_agent<K>(s, knn_lds);
     const uint32_t n_active = s.cell_start[s.n_keys];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t < (int)n_active) orca_reg_agent<K, 4>(s, t);
-    publish_if_last(s);
+    if (t >= (int)n_active) return;
+    knn_agent<K>(s, knn_lds, t);
+    orca_reg_agent<K, 4>(s, t);
 }
 
 template <int K, int OB>
@@ -1873,12 +1893,9 @@ static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStr
     else hipLaunchKernelGGL((k_orca_reg<K, OB>), g, b, 0, st, s);
 }
 
-// Whether the plain step's main ORCA pass runs inside the attraction /
-// k-nearest kernel (k_attr_orca): the launch shape's orca_obst_slots 2
-// selects it for k 5 and 10.  It saves kernel time, but the kernel then
-// publishes the step's replan events only once the main ORCA pass is done
-// too, so the host's replanning overlaps less of the step; the autotuner
-// weighs the two on the live workload.
+// Whether the plain step's main ORCA pass runs inside the k-nearest kernel
+// (k_knn_orca): the launch shape's orca_obst_slots 2 selects it for k 5 and
+// 10, and the autotuner weighs it on the live workload.
 bool orca_fused(int k, int mode, int oslots) { return mode == kPlain && (k == 5 || k == 10) && oslots == 2; }
 
 // Obstacle constraints held in registers by k_lp3d_reg; its persistent
@@ -1891,7 +1908,7 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int
                           hipEvent_t mid) {
     dim3 g(round8(cdiv(s.n_slots, 64))), b(64), gq(kQueueBlocks);
     if (orca_fused(s.k, mode, oslots)) {
-        // the main pass ran inside k_attr_orca
+        // the main pass ran inside k_knn_orca
     } else if (oslots <= 1) launch_orca_main<K, 1>(s, mode, g, b, st);
     else if (oslots <= 2) launch_orca_main<K, 2>(s, mode, g, b, st);
     else launch_orca_main<K, 4>(s, mode, g, b, st);
@@ -1912,17 +1929,21 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int
     }
 }
 
-hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st) {
+// `mid`, when not null, is recorded between the attraction pass and the
+// k-nearest pass.
+hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st, hipEvent_t mid) {
+    hipLaunchKernelGGL(k_attr, dim3(round8(cdiv(s.n_slots, 256))), dim3(256), 0, st, s);
+    if (mid) (void)hipEventRecord(mid, st);
     if (block != 64 && block != 128) block = 256;
     dim3 g(round8(cdiv(s.n_slots, block))), b(block);
     const size_t lds = (size_t)(kKnnBuf + 2 * kKnnRanges) * block * sizeof(int);
     if (orca_fused(s.k, mode, oslots)) {
-        if (s.k == 5) hipLaunchKernelGGL((k_attr_orca<5>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_attr_orca<10>), g, b, lds, st, s);
+        if (s.k == 5) hipLaunchKernelGGL((k_knn_orca<5>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_knn_orca<10>), g, b, lds, st, s);
         return hipGetLastError();
     }
     switch (s.k) {
-#define ECM_KNN_CASE(k) case k: hipLaunchKernelGGL((k_attr_knn<k>), g, b, lds, st, s); break;
+#define ECM_KNN_CASE(k) case k: hipLaunchKernelGGL((k_knn<k>), g, b, lds, st, s); break;
         ECM_KNN_CASE(1) ECM_KNN_CASE(2) ECM_KNN_CASE(3) ECM_KNN_CASE(4) ECM_KNN_CASE(5) ECM_KNN_CASE(6)
         ECM_KNN_CASE(7) ECM_KNN_CASE(8) ECM_KNN_CASE(9) ECM_KNN_CASE(10) ECM_KNN_CASE(11) ECM_KNN_CASE(12)
         ECM_KNN_CASE(13) ECM_KNN_CASE(14) ECM_KNN_CASE(15) ECM_KNN_CASE(16)

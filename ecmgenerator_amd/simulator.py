@@ -289,10 +289,10 @@ class Simulator:
 
     def last_timing(self):
         """[grid build, attraction + k-nearest, learned proposal, ORCA + LP +
-        integration, the main ORCA pass alone] device ms of the last step
-        (with set_timing(True))."""
-        a = (C.c_float * 5)()
-        N.check(self._L.ecm_sim_last_timing(self._h, a, 5))
+        integration, the main ORCA pass alone, the attraction pass alone]
+        device ms of the last step (with set_timing(True))."""
+        a = (C.c_float * 6)()
+        N.check(self._L.ecm_sim_last_timing(self._h, a, 6))
         return list(a)
 
     def set_graph(self, on: bool = True) -> None:

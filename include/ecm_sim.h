@@ -115,8 +115,7 @@ int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
  * 2, 4; default 1): the plain step holds 4 obstacle constraints per agent
  * in registers (8 with orca_obst_slots 4 and k <= 10) and sends agents with
  * more to the overflow pass; with orca_obst_slots 2 and k 5 or 10 that pass
- * runs inside the attraction + k-nearest kernel (less kernel time, but the
- * step's replan events reach the host later); the learned and logging steps
+ * runs inside the k-nearest kernel; the learned and logging steps
  * give each agent orca_obst_slots slots beyond k in an LDS tile, agents
  * with more constraints borrow from their workgroup's small spill area, and
  * those with more obstacle constraints than the tile holds, or whose
@@ -212,10 +211,11 @@ int ecm_sim_stats(ecm_sim* s, uint64_t* out, int n);
  * handle's stream): [0] grid build, [1] attraction + k-nearest, [2] learned
  * proposal (features + network; 0 without one), [3] ORCA + LP +
  * integration, [4] of [3], the main ORCA pass alone (the rest is the
- * passes over its queues: obstacle overflow and the 3D LP).  In the plain
- * step with k 5 or 10 and orca_obst_slots 2 the main ORCA pass runs inside
- * the attraction + k-nearest kernel: [1] includes it and [4] is empty.
- * Enable with ecm_sim_set_timing(s, 1); copies min(n, 5). */
+ * passes over its queues: obstacle overflow and the 3D LP), [5] of [1], the
+ * attraction pass alone (goal checks, IRM, steering; the rest is the
+ * k-nearest pass).  In the plain step with k 5 or 10 and orca_obst_slots 2
+ * the main ORCA pass runs inside the k-nearest kernel: [1] includes it and
+ * [4] is empty.  Enable with ecm_sim_set_timing(s, 1); copies min(n, 6). */
 int ecm_sim_set_timing(ecm_sim* s, int on);
 int ecm_sim_last_timing(ecm_sim* s, float* out, int n);
 
