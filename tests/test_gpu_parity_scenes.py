@@ -41,11 +41,13 @@ def c2_strata(n_gap=700, n_front=700, n_rand=400):
     return choose
 
 
-def run(wl, steps, choose=None, keep=True, seed=0, every=1, learned=None, close=False):
+def run(wl, steps, choose=None, keep=True, seed=0, every=1, learned=None, close=False, shape=None):
     """Step the workload, comparing the GPU with the oracle at `steps` on the
-    agents `choose(state, rng)` picks (every agent when None).  Returns the
-    worst mismatch counts, the run's statistics and per-step tallies of the
-    compared agents (how many the 3D LP decided, obstacle constraints)."""
+    agents `choose(state, rng)` picks (every agent when None), with the
+    launch shape `shape` (cell scale, workgroup, ORCA obstacle slots) when
+    given.  Returns the worst mismatch counts, the run's statistics and
+    per-step tallies of the compared agents (how many the 3D LP decided,
+    obstacle constraints)."""
     from oracle import oracle_py as O
     p = wl.params
     p.keep_orca_velocity = keep
@@ -54,6 +56,9 @@ def run(wl, steps, choose=None, keep=True, seed=0, every=1, learned=None, close=
     assert (ids >= 0).all(), "every generated agent must spawn"
     if learned is not None:
         sim.set_learned(*learned)
+    if shape is not None:
+        sim.set_tuning(*shape)
+        sim.set_graph(True)
     rng = np.random.default_rng(seed)
     worst, tally = {}, []
     for it in range(max(steps) + 1):
@@ -114,9 +119,12 @@ def test_c1_subset_parity():
 def test_c2_stratified_parity_full_size():
     """Bit-exact positions, velocities, ORCA velocities, attraction points,
     neighbour lists and replans on ~1,800 agents per compared step, most of
-    them in or in front of the gaps; the 3D LP decides hundreds of them."""
+    them in or in front of the gaps; the 3D LP decides hundreds of them.  Run
+    with the launch shape the benchmark's autotuner picks at c2 (the main
+    ORCA pass inside the k-nearest kernel) and step graphs."""
     wl = W.build("c2", seed=1)
-    worst, st, tally = run(wl, steps=(0, 1, 10, 100, 150, 299), choose=c2_strata(), keep=True)
+    worst, st, tally = run(wl, steps=(0, 1, 10, 100, 150, 299), choose=c2_strata(), keep=True,
+                           shape=(1.25, 256, 2))
     print(st, flush=True)
     assert all(v == 0 for v in worst.values()), worst
     late = [t for t in tally if t["step"] >= 100]
