@@ -132,15 +132,12 @@ def main():
                 print(f"[bench] prerun step {i + 1}/{prerun} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         sim.sync()
 
-    for _ in range(args.warmup):
-        sim.Update()
-    sim.sync()
     import tempfile
     snap_dir = tempfile.mkdtemp(prefix="ecm_bench_")
     snap = os.path.join(snap_dir, "state.bin")
     sim.snapshot_save(snap)
     # the autotuner times its trials on the live workload (advancing it);
-    # every timed step below starts again from the state after warmup
+    # every timed step below starts again from the state after the prerun
     tuning = None
     if args.autotune > 0:
         tuning = sim.autotune(args.autotune)
@@ -149,9 +146,11 @@ def main():
         cs, kb, os_ = args.launch_shape.split(",")
         sim.set_tuning(cell_scale=float(cs), knn_block=int(kb), orca_obst_slots=int(os_))
         tuning = sim.get_tuning()
-    # both legs time the same steps: the state just loaded, one untimed step
-    # (buffer re-creation after the load), then K timed steps
-    first_ms = _first_step_after_load(sim, snap)
+    # both legs time the same steps: the state just loaded, W untimed warmup
+    # steps (the first re-creates buffers after the load and has no neighbour
+    # seeds; the rest bring the GPU out of its idle state of the file load),
+    # then K timed steps
+    first_ms = _load_and_warm(sim, snap, args.warmup)
     print(f"[bench] plain: first step after load {first_ms:.2f} ms (untimed)", file=sys.stderr, flush=True)
     st0 = sim.stats()
     t0 = time.perf_counter()
@@ -258,7 +257,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": name, **wl.describe(), "agents_spawned": n_spawned,
                    "mean_path_points": round(mean_path, 2),
-                   "timed_window_steps": [prerun + args.warmup + 2, prerun + args.warmup + 1 + args.steps]},
+                   "timed_window_steps": [prerun + max(1, args.warmup) + 1, prerun + max(1, args.warmup) + args.steps]},
         "crowd": {"density_per_m2_1m_grid": density,
                   "lp3d_rate": round((st1["lp3d"] - st0["lp3d"]) / max(1, updates), 5),
                   "active_agents": int(window_state["active"].astype(bool).sum())},
@@ -288,13 +287,19 @@ def _profiled(workload, kernel, what):
     return (t, os.path.relpath(path, ROOT)) if t else (None, None)
 
 
-def _first_step_after_load(sim, snap):
+def _load_and_warm(sim, snap, warmup):
+    """Load the snapshot and run `warmup` (at least one) untimed steps;
+    returns the first one's time in ms."""
     sim.snapshot_load(snap)
     sim.sync()
     t0 = time.perf_counter()
     sim.Update()
     sim.sync()
-    return 1000.0 * (time.perf_counter() - t0)
+    first = 1000.0 * (time.perf_counter() - t0)
+    for _ in range(max(0, warmup - 1)):
+        sim.Update()
+    sim.sync()
+    return first
 
 
 def _default_weights():
@@ -326,7 +331,7 @@ def learned_leg(sim, wl, snap, wpath, args):
     max_dv = float(np.abs(a["velocity"][alive] - b["velocity"][alive]).max() / k_vel) if alive.any() else 0.0
     max_dp = float(np.abs(a["position"][act] - b["position"][act]).max()) if act.any() else 0.0
     # timed: the same K steps as the plain leg
-    first_ms = _first_step_after_load(sim, snap)
+    first_ms = _load_and_warm(sim, snap, args.warmup)
     print(f"[bench] learned: first step after load {first_ms:.2f} ms (untimed)", file=sys.stderr, flush=True)
     st0 = sim.stats()
     t0 = time.perf_counter()
