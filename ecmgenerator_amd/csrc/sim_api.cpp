@@ -7,6 +7,7 @@
 #include "bucket_geom.h"
 #include "mlp_host.h"
 #include "sim_device.h"
+#include "replan_pool.h"
 
 #include <hip/hip_runtime.h>
 
@@ -18,7 +19,10 @@
 #include <cstring>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -37,8 +41,8 @@ hipError_t launch_spawn_scatter(int n, const int* ids, const float2* start, cons
                                 const int* off, const int* cnt, float2* pos, float2* vel, float* radius,
                                 float* pref_speed, float2* pref_vel, uint8_t* active, int* path_off,
                                 int* path_cnt, hipStream_t st);
-hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
-                               int* path_cnt, int2* path_s, hipStream_t st);
+hipError_t launch_path_upload(int n, const int* stage, int npts, const float2* pts, float2* pool_dst, int* path_off,
+                              int* path_cnt, int2* path_s, hipStream_t st);
 hipError_t launch_path_repack(int n_slots, const unsigned long long* acc, const float4* sorted_pr_old,
                               const uint8_t* dead_s_old, int2* path_s_old, int* path_off, const float2* pool,
                               float2* out, uint32_t cap, uint32_t* ctr, hipStream_t st);
@@ -187,13 +191,15 @@ struct ecm_sim {
     uint32_t* h_xfer = nullptr;
     uint32_t* h_xfer_dev = nullptr;          // the same memory as the kernels see it
     uint32_t step_seq = 0;                   // steps launched (matches acc[12])
-    // Replan upload staging, reused step after step; ev_stage marks the end
-    // of the last upload that read it.
-    int* h_stage = nullptr;                  // pinned ids, offsets, counts
+    // Replan upload staging in mapped host memory, which the upload kernel
+    // reads directly; reused step after step, ev_stage marks the end of the
+    // last upload that read it.
+    int* h_stage = nullptr;                  // pinned ids, offsets, counts, sorted positions
+    int* h_stage_dev = nullptr;              // the same memory as the kernels see it
     size_t h_stage_cap = 0;
     float2* h_pts = nullptr;                 // pinned path points
+    float2* h_pts_dev = nullptr;
     size_t h_pts_cap = 0;
-    DBuf<int> d_stage;
     hipEvent_t ev_stage = nullptr;
 
     // environments (0 is created with the handle)
@@ -228,6 +234,7 @@ struct ecm_sim {
 
     // callbacks, stats, timing
     int replan_threads = 1;
+    ReplanPool replan_pool;
     uint64_t stat[ECM_SIM_N_STATS] = {};
     bool timing = false;
     // [5]: after the main ORCA pass, [6]: after the attraction pass
@@ -304,7 +311,7 @@ static void free_all(ecm_sim* s) {
     for (int b = 0; b < 2; ++b) { s->nbr[b].release(); s->nbr_cnt[b].release(); }
     s->fwd.release(); s->prev_of.release(); s->pos_of.release();
     s->new_ids.release(); s->dead_ids.release();
-    s->xfer.release(); s->wstats.release(); s->acc.release(); s->d_stage.release();
+    s->xfer.release(); s->wstats.release(); s->acc.release();
     s->q3d_agent.release(); s->ovf_agent.release(); s->q3d_info.release(); s->q3d_vel.release(); s->q3d_con.release(); s->q3d_big.release();
     for (auto& e : s->envs) e->release();
     s->d_envs.release();
@@ -317,8 +324,8 @@ static void free_all(ecm_sim* s) {
     s->repack_ctr.release();
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     if (s->h_pts) (void)hipHostFree(s->h_pts);
-    s->h_stage = nullptr;
-    s->h_pts = nullptr;
+    s->h_stage = s->h_stage_dev = nullptr;
+    s->h_pts = s->h_pts_dev = nullptr;
     s->h_stage_cap = s->h_pts_cap = 0;
     s->feat.release();
     s->mlp_mask.release();
@@ -1384,27 +1391,24 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     std::vector<std::vector<float>> res(n);
     std::vector<int> cnt(n, 0);
     auto work = [&](int i) {
+        // the planner writes into a per-thread buffer of cap points, made
+        // once per thread; only the points it returns are kept
+        thread_local std::vector<float> buf;
+        if (buf.size() < 2 * (size_t)cap) buf.resize(2 * (size_t)cap);
         int id = rid[i];
-        res[i].resize(2 * cap);
         float2 g = s->h_goal[id];
         int np = 0;
         const EnvHost* E = s->envs[s->h_env[id]].get();
         int r = E->replan_fn ? E->replan_fn(E->replan_user, id, rpos[i].x, rpos[i].y, g.x, g.y, s->h_radius[id],
-                                            res[i].data(), cap, &np)
+                                            buf.data(), cap, &np)
                              : 1;
-        cnt[i] = (r == 0 && np >= 1) ? np : 0;     // an empty result keeps the old path
+        // an empty result, or one that claims more than cap points, keeps the old path
+        cnt[i] = (r == 0 && np >= 1 && np <= cap) ? np : 0;
+        if (cnt[i]) res[i].assign(buf.begin(), buf.begin() + 2 * (size_t)cnt[i]);
     };
-    int nt = std::min(std::max(1, s->replan_threads), n);
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) work(i);
-    } else {
-        std::atomic<int> next{0};
-        std::vector<std::thread> th;
-        for (int t = 1; t < nt; ++t)
-            th.emplace_back([&]() { for (int i; (i = next++) < n;) work(i); });
-        for (int i; (i = next++) < n;) work(i);
-        for (auto& t : th) t.join();
-    }
+    // an item that threw (host memory exhausted) keeps its old path, as a
+    // planner failure does
+    (void)s->replan_pool.run(std::max(1, s->replan_threads), n, work);
     std::vector<int> ids, offs, cnts, ats;
     std::vector<float2> pts;
     for (int i = 0; i < n; ++i) {
@@ -1431,41 +1435,40 @@ static int run_replans(ecm_sim* s, const std::vector<int>& rid, const std::vecto
     // sync: wait for the copies of the last upload that read it (long done
     // in the usual case), and grow it only when the stream is idle.
     HIPCHK(hipEventSynchronize(s->ev_stage));
-    if (s->h_stage_cap < 4 * m || s->h_pts_cap < pts.size() || s->d_stage.n < 4 * m) {
+    if (s->h_stage_cap < 4 * m || s->h_pts_cap < pts.size()) {
         if ((rc = settle(s)) != 0) return rc;
+        const unsigned flags = hipHostMallocMapped | hipHostMallocCoherent;
         if (s->h_stage_cap < 4 * m) {
             if (s->h_stage) (void)hipHostFree(s->h_stage);
-            s->h_stage = nullptr;
-            s->h_stage_cap = std::max<size_t>(4 * m, 4 * 1024);
-            if (hipHostMalloc((void**)&s->h_stage, s->h_stage_cap * sizeof(int)) != hipSuccess) {
-                s->h_stage_cap = 0;
-                return fail(ECM_E_HIP, "hipHostMalloc failed");
-            }
+            s->h_stage = s->h_stage_dev = nullptr;
+            s->h_stage_cap = 0;
+            const size_t cap = std::max<size_t>(4 * m, 4 * 1024);
+            if (hipHostMalloc((void**)&s->h_stage, cap * sizeof(int), flags) != hipSuccess ||
+                hipHostGetDevicePointer((void**)&s->h_stage_dev, s->h_stage, 0) != hipSuccess)
+                return fail(ECM_E_HIP, "hipHostMalloc (replan staging) failed");
+            s->h_stage_cap = cap;
         }
         if (s->h_pts_cap < pts.size()) {
             if (s->h_pts) (void)hipHostFree(s->h_pts);
-            s->h_pts = nullptr;
-            s->h_pts_cap = std::max<size_t>(pts.size(), 32 * 1024);
-            if (hipHostMalloc((void**)&s->h_pts, s->h_pts_cap * sizeof(float2)) != hipSuccess) {
-                s->h_pts_cap = 0;
-                return fail(ECM_E_HIP, "hipHostMalloc failed");
-            }
+            s->h_pts = s->h_pts_dev = nullptr;
+            s->h_pts_cap = 0;
+            const size_t cap = std::max<size_t>(pts.size(), 32 * 1024);
+            if (hipHostMalloc((void**)&s->h_pts, cap * sizeof(float2), flags) != hipSuccess ||
+                hipHostGetDevicePointer((void**)&s->h_pts_dev, s->h_pts, 0) != hipSuccess)
+                return fail(ECM_E_HIP, "hipHostMalloc (replan staging) failed");
+            s->h_pts_cap = cap;
         }
-        if (s->d_stage.n < 4 * m && (rc = s->d_stage.alloc(std::max<size_t>(4 * m, 4 * 1024))) != 0) return rc;
     }
     std::memcpy(s->h_stage, ids.data(), m * sizeof(int));
     std::memcpy(s->h_stage + m, offs.data(), m * sizeof(int));
     std::memcpy(s->h_stage + 2 * m, cnts.data(), m * sizeof(int));
     std::memcpy(s->h_stage + 3 * m, ats.data(), m * sizeof(int));
     std::memcpy(s->h_pts, pts.data(), pts.size() * sizeof(float2));
-    HIPCHK(hipMemcpyAsync(s->path_pool.p + s->path_used, s->h_pts, pts.size() * sizeof(float2), hipMemcpyHostToDevice,
-                          s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_stage.p, s->h_stage, 4 * m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipEventRecord(s->ev_stage, s->stream));
     // the step that asked for them is this one ([cur]): its sorted copy is
     // what the next step carries in
-    HIPCHK(launch_path_scatter((int)m, s->d_stage.p, s->d_stage.p + m, s->d_stage.p + 2 * m, s->d_stage.p + 3 * m,
-                               s->path_off.p, s->path_cnt.p, s->path_s[s->cur].p, s->stream));
+    HIPCHK(launch_path_upload((int)m, s->h_stage_dev, (int)pts.size(), s->h_pts_dev, s->path_pool.p + s->path_used,
+                              s->path_off.p, s->path_cnt.p, s->path_s[s->cur].p, s->stream));
+    HIPCHK(hipEventRecord(s->ev_stage, s->stream));
     s->path_used += pts.size();
     return 0;
 }

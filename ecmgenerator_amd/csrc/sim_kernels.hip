@@ -1999,23 +1999,32 @@ __global__ void k_spawn_scatter(int n, const int* ids, const float2* start, cons
     path_cnt[id] = cnt[i];
 }
 
-// UpdatePath results for a batch of agents (Simulator.cpp:114-123).  With
-// `at` (each agent's position in the last step's sorted outputs), the copy
-// the next step carries in Morton order is updated too.
-__global__ void k_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
-                               int* path_cnt, int2* path_s) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    path_off[ids[i]] = off[i];
-    path_cnt[ids[i]] = cnt[i];
-    if (at) path_s[at[i]] = make_int2(off[i], cnt[i]);
+// UpdatePath results for a batch of agents (Simulator.cpp:114-123), read
+// directly from the host's mapped staging (one kernel instead of two copies
+// and a scatter): the npts new points go to the pool at pool_dst, and each
+// agent's offset and count to the slot arrays and to the copy the next step
+// carries in Morton order (path_s, at the agent's position in the last
+// step's sorted outputs).  stage holds n ids, n offsets, n counts, then n
+// sorted positions.  Grid-stride loops: any grid covers both lists.
+__global__ void k_path_upload(int n, const int* stage, int npts, const float2* pts, float2* pool_dst, int* path_off,
+                              int* path_cnt, int2* path_s) {
+    const int stride = gridDim.x * blockDim.x;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = i0; i < npts; i += stride) pool_dst[i] = pts[i];
+    for (int i = i0; i < n; i += stride) {
+        const int id = stage[i], off = stage[n + i], cnt = stage[2 * n + i], at = stage[3 * n + i];
+        path_off[id] = off;
+        path_cnt[id] = cnt;
+        path_s[at] = make_int2(off, cnt);
+    }
 }
 
-hipError_t launch_path_scatter(int n, const int* ids, const int* off, const int* cnt, const int* at, int* path_off,
-                               int* path_cnt, int2* path_s, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_path_scatter, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ids, off, cnt, at, path_off, path_cnt,
-                       path_s);
+hipError_t launch_path_upload(int n, const int* stage, int npts, const float2* pts, float2* pool_dst, int* path_off,
+                              int* path_cnt, int2* path_s, hipStream_t st) {
+    if (n <= 0 && npts <= 0) return hipSuccess;
+    const int blocks = std::min(cdiv(std::max(n, npts), 256), 64);
+    hipLaunchKernelGGL(k_path_upload, dim3(blocks), dim3(256), 0, st, n, stage, npts, pts, pool_dst, path_off,
+                       path_cnt, path_s);
     return hipGetLastError();
 }
 
