@@ -1899,9 +1899,14 @@ static void launch_orca_main(const DevState& s, int mode, dim3 g, dim3 b, hipStr
 bool orca_fused(int k, int mode, int oslots) { return mode == kPlain && (k == 5 || k == 10) && oslots == 2; }
 
 // Obstacle constraints held in registers by k_lp3d_reg; its persistent
-// grid is kLp3dRegBlocks workgroups of 256 per CU.
+// grid is kLp3dRegBlocks workgroups of 256 per CU, as many as its 4-wave
+// cap keeps resident, so the whole grid starts at once and each wavefront
+// loops over its share of the queue.  At c2's jam (560k entries) that took
+// the queue passes from 0.153 to 0.147 ms; 2, 3, 6 and 8 workgroups per
+// CU measured 0.208, 0.165, 0.175 and 0.153 ms, and handing the chunks out
+// through an atomic counter 0.21-0.26 ms.
 constexpr int kLp3dRegObst = 4;
-constexpr int kLp3dRegBlocks = 8;
+constexpr int kLp3dRegBlocks = 4;
 
 template <int K>
 static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int n_cu, hipStream_t st,

@@ -96,11 +96,14 @@ typedef int (*ecm_replan_fn)(void* user, int id, float x, float y, float goal_x,
 int ecm_sim_set_replan_callback(ecm_sim* s, ecm_replan_fn fn, void* user);
 /* Threads that may call the replan callback concurrently within one step
  * (default 1).  Only raise it for a thread-safe callback, such as
- * ecm_scene_replan_callback(). */
+ * ecm_scene_replan_callback().  With n > 1 the handle keeps n - 1 worker
+ * threads (made at the first step that replans, remade when n changes,
+ * joined by ecm_sim_destroy); the stepping thread is the n-th. */
 int ecm_sim_set_replan_threads(ecm_sim* s, int n);
 
-/* Step graphs: with on, the device work of a step is captured into
- * hipGraphs (two per step: up to the replan read-back, and the rest) and
+/* Step graphs: with on, the device work of a step is captured into one
+ * hipGraph (one per position-buffer parity; the host reads the replan
+ * events from host-mapped memory while the graph still runs) and
  * replayed, recaptured whenever buffers, parameters or the launch shape
  * change.  Steps with timing on run eagerly.  If capture fails the handle
  * falls back to eager launches; ecm_sim_graph_status reports whether
@@ -110,7 +113,7 @@ int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
 
 /* Launch shape; no choice changes any result.  cell_scale multiplies the
  * agent grid cell edge (about one agent per cell at 1; default 1.25),
- * knn_block is the workgroup size of the attraction + k-nearest kernel (64,
+ * knn_block is the workgroup size of the k-nearest kernel (64,
  * 128, 256; default 256), and orca_obst_slots sizes the main ORCA pass (1,
  * 2, 4; default 1): the plain step holds 4 obstacle constraints per agent
  * in registers (8 with orca_obst_slots 4 and k <= 10) and sends agents with
