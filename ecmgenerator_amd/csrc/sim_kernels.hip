@@ -1287,27 +1287,42 @@ __device__ void agent_features(const DevState& s, const EnvDesc& E, int t, V2 po
                                      pack_bf16x2(f[8 * c + 4], f[8 * c + 5]), pack_bf16x2(f[8 * c + 6], f[8 * c + 7]));
 }
 
+// The obstacle bucket of position: entries [e0, e1) of obucket_edges, the
+// candidate edges in ascending id order.
+__device__ __forceinline__ void obstacle_bucket(const EnvDesc& s, V2 position, int& e0, int& e1) {
+    int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
+    int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
+    int b = by * s.og.side + bx;
+    e0 = s.obucket_start[b];
+    e1 = s.obucket_start[b + 1];
+}
+// Whether bucket entry q is an obstacle neighbour of position (the test of
+// Simulator.cpp:266-287); e receives its edge, dd the squared distance to
+// the edge's closest point.
+__device__ __forceinline__ bool obstacle_near(const EnvDesc& s, V2 position, float range2, int q, int& e,
+                                              float& dd) {
+    e = s.obucket_edges[q];
+    float4 a = s.oe0[e];
+    V2 L = mk(a.x, a.y), R = mk(a.z, a.w);
+    float sld = det(L - position, R - L);
+    float sqd = sld * sld / sq_dist(L, R);
+    if (!(sqd < range2 && sld < 0.0f)) return false;
+    V2 c = closest_on_segment(position, L, R);
+    dd = sq_dist(c, position);
+    return dd < range2;
+}
 // Obstacle query with the reference's filter (Simulator.cpp:266-287) over
 // the agent's bucket; `emit(e, dd)` runs for every edge it keeps, in scan
 // order, and returns false to stop the scan.
 template <typename F>
 __device__ __forceinline__ void obstacle_query(const EnvDesc& s, V2 position, float range2, F&& emit) {
-    int bx = grid_cell(position.x, s.og.x0, s.og.inv_cell, s.og.side);
-    int by = grid_cell(position.y, s.og.y0, s.og.inv_cell, s.og.side);
-    int b = by * s.og.side + bx;
-    int e0 = s.obucket_start[b], e1 = s.obucket_start[b + 1];
+    int e0, e1;
+    obstacle_bucket(s, position, e0, e1);
     for (int q = e0; q < e1; ++q) {
-        int e = s.obucket_edges[q];
-        float4 a = s.oe0[e];
-        V2 L = mk(a.x, a.y), R = mk(a.z, a.w);
-        float sld = det(L - position, R - L);
-        float sqd = sld * sld / sq_dist(L, R);
-        if (sqd < range2 && sld < 0.0f) {
-            V2 c = closest_on_segment(position, L, R);
-            const float dd = sq_dist(c, position);
-            if (dd < range2)
-                if (!emit(e, dd)) return;
-        }
+        int e;
+        float dd;
+        if (obstacle_near(s, position, range2, q, e, dd))
+            if (!emit(e, dd)) return;
     }
 }
 
@@ -1700,14 +1715,95 @@ __device__ __forceinline__ void orca_reg_agent(const DevState& s, int t) {
     integrate(s, t, position, velocity, o);
 }
 
+// One overflow agent per wavefront (64-thread block): the lanes build its
+// constraints together, and lane 0 runs the 2D LP over them.  The obstacle
+// constraints of the bucket's edges are built one edge per lane and kept in
+// bucket order (a ballot prefix gives each its slot), the order
+// obstacle_query emits them in; the agent constraints are built one
+// neighbour per lane.  Each constraint's arithmetic, the LP and what follows
+// it are those of orca_agent in the plain mode, so the result is the same;
+// a lane working through an agent alone waits out every constraint's
+// loads and arithmetic in turn.  lds holds MAXO + K constraints at slot c.
+template <int K, int MAXO>
+__device__ __forceinline__ void orca_agent_wave(const DevState& s, int t, float4* lds) {
+    const int lane = threadIdx.x;
+    const Tile T{0, 0, 0};                           // slot c at lds[c]
+    const float4 pr = s.sorted_pr[t];
+    const float4 sv = s.sorted_v[t];
+    const float2 v0 = make_float2(sv.x, sv.y);
+    if (s.dead_s[t]) {                               // destroyed this step: state unchanged
+        if (lane == 0) {
+            s.pos_s[t] = make_float2(pr.x, pr.y);
+            s.vel_s[t] = v0;
+        }
+        return;
+    }
+    const V2 position = mk(pr.x, pr.y);
+    const V2 velocity = mk(v0.x, v0.y);
+    const float rad = pr.z;
+    const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
+    const EnvDesc& E = env_of_sorted(s, sv);
+    int nObst = 0;                                   // the same in every lane
+    if (E.n_obst_edges > 0) {
+        const float range = s.time_horizon_obst * maxSpeed + rad;          // ORCA.cpp:27
+        int e0, e1;
+        obstacle_bucket(E, position, e0, e1);
+        for (int q0 = e0; q0 < e1; q0 += 64) {
+            const int q = q0 + lane;
+            bool made = false;
+            V2 cp = mk(0.0f, 0.0f), cn = mk(0.0f, 0.0f);
+            int e;
+            float dd;
+            if (q < e1 && obstacle_near(E, position, range * range, q, e, dd))
+                made = obstacle_constraint(s, E, e, position, velocity, rad, cp, cn);
+            const unsigned long long m = __ballot(made);
+            const int slot = nObst + __popcll(m & ((1ull << lane) - 1ull));
+            if (made && slot < MAXO) con_st(lds, T, slot, cp, cn);
+            nObst += __popcll(m);
+        }
+        nObst = min(nObst, MAXO);                    // MAXO covers the longest bucket
+    }
+    const int cnt = s.nbr_cnt[t];
+    if (lane < cnt) {
+        const int sj = s.nbr[(size_t)lane * s.n_slots + t];
+        V2 cp, cn;
+        agent_constraint(s, position, velocity, rad, s.sorted_pr[sj], sorted_vel(s, sj), cp, cn);
+        con_st(lds, T, nObst + lane, cp, cn);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const int nc = nObst + cnt;
+        const float2 pv = s.pref_vel_s[t];
+        V2 o;
+        const int fail = lp2d(lds, T, nc, nc, 0, mk(pv.x, pv.y), maxSpeed, false, o);
+        if (fail < nc) {
+            // RandomizedLP3D runs on the compacted queue, as from orca_agent
+            const uint32_t qi = wave_alloc(&s.stats[0]);
+            s.q3d_agent[qi] = t;
+            s.q3d_info[qi] = make_int4(nc, nObst, fail, 0);
+            s.q3d_vel[qi] = make_float2(o.x, o.y);
+            for (int c = 0; c < nc; ++c) *q3d_slot(s, qi, c) = lds[c];
+        } else {
+            integrate(s, t, position, velocity, o);
+        }
+    }
+    __syncthreads();                                 // lds is reused by the next agent
+}
+
 // Agents with more obstacle constraints than the main tile holds, with a tile for
 // MAXO (>= the longest obstacle bucket list), plain LP.  Fixed grid,
-// strided loop.
+// strided loop.  A queue no longer than the grid (the usual case: a few
+// hundred agents of millions) gets one agent per wavefront
+// (orca_agent_wave); a longer one, one agent per lane.
 template <int K, int MAXO>
 __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     __shared__ float4 lds[(MAXO + K) * 64];
     const uint32_t nq = s.stats[1];
     const int tid = threadIdx.x;
+    if (nq <= gridDim.x) {
+        if (blockIdx.x < nq) orca_agent_wave<K, MAXO>(s, s.ovf_agent[blockIdx.x], lds);
+        return;
+    }
     // entries go lane-major across the blocks (entry i: lane i / gridDim,
     // block i % gridDim), so a short queue spreads over many CUs instead of
     // filling the lanes of a few divergent wavefronts

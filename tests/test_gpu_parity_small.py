@@ -90,3 +90,40 @@ def test_parity_pillar_forest_obstacle_overflow():
     assert st["lp3d"] > 0 and st["lp3d"] == used_3d, (st, used_3d)
     assert st["lp3d_big"] > 0, st
     assert all(v == 0 for v in worst.values()), worst
+
+
+def test_parity_pillar_forest_long_overflow_queue():
+    """The same pillar lattice over 120 m with 6,000 agents: most of them
+    have more than 4 obstacle constraints at the start, so the overflow
+    queue holds thousands of agents, more than the overflow kernel's grid
+    has wavefronts, and the kernel works one agent per lane instead of one
+    agent per wavefront (the short queues of the test above).  Every agent
+    must match the oracle bit for bit."""
+    from oracle import oracle_py as O
+    from ecmgenerator_amd import scenarios as S
+    half = 60.0
+    box = (-half, -half, half, half)
+    rects = [(x, y, x + 1, y + 1) for x in np.arange(-half + 5, half - 5, 3.5)
+             for y in np.arange(-half + 5, half - 5, 3.5)]
+    obs = S.rects_to_obstacles(rects)
+    ecm = S.empty_square_ecm(*box)
+    rng = np.random.default_rng(5)
+    agents = S.straight_agents(rng, box, 6000, rects, margin=0.4)
+    p = Params(max_agents=6016, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    assert (sim.SpawnAgents(agents) >= 0).sum() > 5000    # overlapping starts are rejected
+    worst = {}
+    longest = 0
+    for it in range(3):
+        before = oracle_state(sim)
+        ora = O.step(p, before, obs, ecm)
+        ovf0 = sim.stats()["obst_overflow"]
+        sim.Update()
+        res = compare_step(sim, before, ora)
+        longest = max(longest, sim.stats()["obst_overflow"] - ovf0)
+        print(f"pillars-120m step={it} {res} overflow queue so far {longest}", flush=True)
+        for k, v in res.items():
+            worst[k] = max(worst.get(k, 0), v)
+    sim.close()
+    assert longest > 1024, longest          # the kernel's grid: 1,024 wavefronts
+    assert all(v == 0 for v in worst.values()), worst
