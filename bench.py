@@ -172,6 +172,21 @@ def main():
         kt += np.array(sim.last_timing())
     sim.set_timing(False)
     updates = st1["agent_updates"] - st0["agent_updates"]
+    # beside (never as) the value: the rate when every step is followed by
+    # GetPositionData, as a renderer drawing every step calls it; that read
+    # waits for the step, brings the positions into slot order (k_unsort)
+    # and copies every slot's position to the host over PCIe
+    n_rb = max(3, min(args.steps, 10))
+    sim.sync()
+    t2 = time.perf_counter()
+    for _ in range(n_rb):
+        sim.Update()
+        pos_host = sim.GetPositionData()
+    t3 = time.perf_counter()
+    readback = {"ms_per_step": round(1000.0 * (t3 - t2) / n_rb, 4),
+                "value": round((updates / args.steps) * n_rb / (t3 - t2), 1),
+                "bytes_to_host_per_step": int(pos_host.nbytes), "steps": n_rb,
+                "what": "each step followed by GetPositionData (slot-order pass + device-to-host copy)"}
     wall = t1 - t0
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / wall
@@ -263,6 +278,7 @@ def main():
                   "active_agents": int(window_state["active"].astype(bool).sum())},
         "launch": {"step_graphs": graph_on, "graph_fallback_reason": graph_why or None, "autotune": tuning},
         "roofline": roof,
+        "with_position_readback": readback,
         "cpu_baseline": cpu,
         "learned": learned,
         "stats": {k: st1[k] - st0[k] for k in st1},
