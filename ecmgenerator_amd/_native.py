@@ -67,6 +67,8 @@ SIGNATURES = {
     "ecm_sim_read": (_I, [_P, _I, _P, C.c_size_t]),
     "ecm_sim_write": (_I, [_P, _I, _P, C.c_size_t]),
     "ecm_sim_device_ptr": (_I, [_P, _I, C.POINTER(_P)]),
+    "ecm_host_alloc": (_I, [C.c_size_t, C.POINTER(_P)]),
+    "ecm_host_free": (_I, [_P]),
     "ecm_sim_num_agents": (_I, [_P, _PI]),
     "ecm_sim_last_index": (_I, [_P, _PI]),
     "ecm_sim_get_neighbors": (_I, [_P, _PI, _PI]),

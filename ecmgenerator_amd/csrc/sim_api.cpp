@@ -1913,6 +1913,21 @@ extern "C" int ecm_sim_write(ecm_sim* s, int c, const void* src, size_t bytes) {
     return 0;
 }
 
+extern "C" int ecm_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(ECM_E_ARG, "null argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(ECM_E_HIP, "hipHostMalloc of %zu bytes failed", bytes);
+    }
+    return 0;
+}
+
+extern "C" int ecm_host_free(void* p) {
+    if (p && hipHostFree(p) != hipSuccess) return fail(ECM_E_HIP, "hipHostFree failed");
+    return 0;
+}
+
 extern "C" int ecm_sim_device_ptr(ecm_sim* s, int c, void** out) {
     if (!s || !out) return fail(ECM_E_ARG, "null argument");
     (void)hipSetDevice(s->dev);

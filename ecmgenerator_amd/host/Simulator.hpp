@@ -50,6 +50,38 @@ struct SpeedComponent { float speed; };
 // advances it (Simulator.cpp:104,118).
 struct PathComponent { int currentIndex; int numPoints; float* x; float* y; };
 
+// A host mirror of one component array in page-locked memory
+// (ecm_host_alloc), so that reading it back every step copies at full PCIe
+// speed.  Trivially copyable elements only; not copyable itself.
+template <typename T>
+class PinnedArray {
+public:
+    PinnedArray() = default;
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    ~PinnedArray() { ecm_host_free(m_Data); }
+    void resize(size_t n) {
+        if (n == m_Size) return;
+        void* p = nullptr;
+        if (ecm_host_alloc(n * sizeof(T), &p) != 0)
+            throw std::runtime_error(std::string("ecm_host_alloc: ") + ecm_last_error());
+        if (m_Data) std::memcpy(p, m_Data, (n < m_Size ? n : m_Size) * sizeof(T));
+        if (n > m_Size) std::memset(static_cast<T*>(p) + m_Size, 0, (n - m_Size) * sizeof(T));
+        ecm_host_free(m_Data);
+        m_Data = static_cast<T*>(p);
+        m_Size = n;
+    }
+    T* data() { return m_Data; }
+    const T* data() const { return m_Data; }
+    size_t size() const { return m_Size; }
+    T& operator[](size_t i) { return m_Data[i]; }
+    const T& operator[](size_t i) const { return m_Data[i]; }
+
+private:
+    T* m_Data = nullptr;
+    size_t m_Size = 0;
+};
+
 // Area.h:11-103
 enum SimAreaType { NONE, WALKABLE, SPAWN, GOAL, OBSTACLE };
 struct SpawnConfiguration {
@@ -106,7 +138,8 @@ class Simulator {
         const size_t N = (size_t)m_MaxNumEntities;
         m_Positions.resize(N); m_Velocities.resize(N); m_PreferredVelocities.resize(N);
         m_AttractionPoints.resize(N); m_Clearances.resize(N); m_Active.resize(N);
-        m_Dirty = m_PathsDirty = true;
+        m_Stale = kAllMirrors;
+        m_PathsDirty = true;
     }
 
     // Simulator::SpawnAgent (Simulator.cpp:168-200): plans the path with the
@@ -114,22 +147,28 @@ class Simulator {
     // location invalid or no path exists.
     int SpawnAgent(float sx, float sy, float gx, float gy, float clearance, float preferredSpeed) {
         const int id = SpawnOne(sx, sy, gx, gy, clearance, preferredSpeed, 1);
-        m_Dirty = m_PathsDirty = true;
+        m_Stale = kAllMirrors;
+        m_PathsDirty = true;
         return id;
     }
     // DestroyAgent (Simulator.cpp:202-208)
-    void DestroyAgent(int idx) { check(ecm_sim_destroy_agent(m_Sim, idx), "destroy"); m_Dirty = m_PathsDirty = true; }
+    void DestroyAgent(int idx) {
+        check(ecm_sim_destroy_agent(m_Sim, idx), "destroy");
+        m_Stale = kAllMirrors;
+        m_PathsDirty = true;
+    }
     // Update (Simulator.cpp:314-323): spawn areas first, as the reference
     void Update(float /*dt*/) {
         UpdateSpawnAreas();
         check(ecm_sim_step(m_Sim, 1), "step");
-        m_Dirty = m_PathsDirty = true;
+        m_Stale = kAllMirrors;
+        m_PathsDirty = true;
     }
 
     // ValidSpawnLocation (Simulator.cpp:295-311): no active agent closer than
     // the clearance
     bool ValidSpawnLocation(float x, float y, float clearance) {
-        sync();
+        sync(kPosMirror | kActiveMirror);
         return ValidOnMirror(x, y, clearance);
     }
 
@@ -225,7 +264,7 @@ class Simulator {
         // the host mirror is read once; agents spawned here are added to its
         // positions and flags as they are placed
         bool spawned = false;
-        if (!m_SpawnAreas.empty()) sync();
+        if (!m_SpawnAreas.empty()) sync(kPosMirror | kActiveMirror);
         for (auto& kv : m_SpawnAreas) {
             SpawnArea& area = kv.second;
             for (size_t ga = 0; ga < area.connectedGoalAreas.size(); ++ga) {
@@ -257,20 +296,29 @@ class Simulator {
                 area.timeSinceLastSpawn[ga] -= (float)agentsToSpawn / rate;
             }
         }
-        if (spawned) m_Dirty = m_PathsDirty = true;
+        if (spawned) {
+            m_Stale = kAllMirrors;
+            m_PathsDirty = true;
+        }
     }
     // Reset (Simulator.cpp:325-333)
-    void Reset() { check(ecm_sim_reset(m_Sim), "reset"); m_Dirty = m_PathsDirty = true; }
+    void Reset() {
+        check(ecm_sim_reset(m_Sim), "reset");
+        m_Stale = kAllMirrors;
+        m_PathsDirty = true;
+    }
 
     // getters (Simulator.h:114-126)
     int GetNumAgents() const { int n = 0; check(ecm_sim_num_agents(m_Sim, &n), "num_agents"); return n; }
     int GetLastIndex() const { int n = 0; check(ecm_sim_last_index(m_Sim, &n), "last_index"); return n; }
-    PositionComponent* GetPositionData() { sync(); return m_Positions.data(); }
-    VelocityComponent* GetVelocityData() { sync(); return m_Velocities.data(); }
-    VelocityComponent* GetPreferredVelocityData() { sync(); return m_PreferredVelocities.data(); }
-    PositionComponent* GetAttractionPointData() { sync(); return m_AttractionPoints.data(); }
-    ClearanceComponent* GetClearanceData() { sync(); return m_Clearances.data(); }
-    bool* GetActiveFlags() { sync(); return reinterpret_cast<bool*>(m_Active.data()); }
+    // each getter reads back only its own component, once per step (a
+    // renderer that draws positions every step moves 8 B per slot, not 37)
+    PositionComponent* GetPositionData() { sync(kPosMirror); return m_Positions.data(); }
+    VelocityComponent* GetVelocityData() { sync(kVelMirror); return m_Velocities.data(); }
+    VelocityComponent* GetPreferredVelocityData() { sync(kPrefVelMirror); return m_PreferredVelocities.data(); }
+    PositionComponent* GetAttractionPointData() { sync(kAttrMirror); return m_AttractionPoints.data(); }
+    ClearanceComponent* GetClearanceData() { sync(kClearMirror); return m_Clearances.data(); }
+    bool* GetActiveFlags() { sync(kActiveMirror); return reinterpret_cast<bool*>(m_Active.data()); }
     // GetPathData (Simulator.h:119): one PathComponent per slot, read back
     // with ecm_sim_read_paths when a spawn, replan or removal may have
     // changed a path since the last read.
@@ -305,7 +353,7 @@ class Simulator {
 
     // Simulator::AddPosition (Simulator.h:87-90)
     void AddPosition(Entity e, float x, float y) {
-        sync();
+        sync(kPosMirror);
         m_Positions[e].x = x;
         m_Positions[e].y = y;
         check(ecm_sim_write(m_Sim, ECM_C_POSITION, m_Positions.data(), m_Positions.size() * sizeof(PositionComponent)),
@@ -316,17 +364,20 @@ class Simulator {
     static void check(int rc, const char* what) {
         if (rc != 0) throw std::runtime_error(std::string("ecm_sim ") + what + ": " + ecm_last_error());
     }
-    void sync() {
-        if (!m_Dirty || !m_Sim) return;
+    // Reads back the components of `want` whose host mirror is stale.
+    void sync(unsigned want) {
+        const unsigned need = want & m_Stale;
+        if (!need || !m_Sim) return;
         const size_t N = (size_t)m_MaxNumEntities;
-        check(ecm_sim_read(m_Sim, ECM_C_POSITION, m_Positions.data(), N * 8), "read");
-        check(ecm_sim_read(m_Sim, ECM_C_VELOCITY, m_Velocities.data(), N * 8), "read");
-        check(ecm_sim_read(m_Sim, ECM_C_PREF_VELOCITY, m_PreferredVelocities.data(), N * 8), "read");
-        check(ecm_sim_read(m_Sim, ECM_C_ATTRACTION, m_AttractionPoints.data(), N * 8), "read");
-        check(ecm_sim_read(m_Sim, ECM_C_CLEARANCE, m_Clearances.data(), N * 4), "read");
-        check(ecm_sim_read(m_Sim, ECM_C_ACTIVE, m_Active.data(), N), "read");
-        check(ecm_sim_last_index(m_Sim, &m_LastIdx), "last_index");
-        m_Dirty = false;
+        if (need & kPosMirror) check(ecm_sim_read(m_Sim, ECM_C_POSITION, m_Positions.data(), N * 8), "read");
+        if (need & kVelMirror) check(ecm_sim_read(m_Sim, ECM_C_VELOCITY, m_Velocities.data(), N * 8), "read");
+        if (need & kPrefVelMirror)
+            check(ecm_sim_read(m_Sim, ECM_C_PREF_VELOCITY, m_PreferredVelocities.data(), N * 8), "read");
+        if (need & kAttrMirror) check(ecm_sim_read(m_Sim, ECM_C_ATTRACTION, m_AttractionPoints.data(), N * 8), "read");
+        if (need & kClearMirror) check(ecm_sim_read(m_Sim, ECM_C_CLEARANCE, m_Clearances.data(), N * 4), "read");
+        if (need & kActiveMirror) check(ecm_sim_read(m_Sim, ECM_C_ACTIVE, m_Active.data(), N), "read");
+        if (m_Stale == kAllMirrors) check(ecm_sim_last_index(m_Sim, &m_LastIdx), "last_index");
+        m_Stale &= ~need;
     }
     void ClearSimulator() {
         if (m_Sim) ecm_sim_destroy(m_Sim);
@@ -382,14 +433,17 @@ class Simulator {
     int m_MaxNumEntities;
     ecm_sim_params m_Params{};
     ecm_sim* m_Sim = nullptr;
-    bool m_Dirty = true;
+    // host mirrors not read back since the simulation last changed
+    static constexpr unsigned kPosMirror = 1u, kVelMirror = 2u, kPrefVelMirror = 4u, kAttrMirror = 8u,
+                              kClearMirror = 16u, kActiveMirror = 32u, kAllMirrors = 63u;
+    unsigned m_Stale = kAllMirrors;
     bool m_PathsDirty = true;
     std::vector<PathComponent> m_Paths;
     std::vector<float> m_PathX, m_PathY;
-    std::vector<PositionComponent> m_Positions, m_AttractionPoints;
-    std::vector<VelocityComponent> m_Velocities, m_PreferredVelocities;
-    std::vector<ClearanceComponent> m_Clearances;
-    std::vector<unsigned char> m_Active;
+    PinnedArray<PositionComponent> m_Positions, m_AttractionPoints;
+    PinnedArray<VelocityComponent> m_Velocities, m_PreferredVelocities;
+    PinnedArray<ClearanceComponent> m_Clearances;
+    PinnedArray<unsigned char> m_Active;
     std::map<int, SpawnArea> m_SpawnAreas;
     std::map<int, GoalArea> m_GoalAreas;
     std::vector<ObstacleArea> m_ObstacleAreas;

@@ -119,6 +119,7 @@ class Simulator:
         self.SetEcm(ecm)
         self._planner = planner
         self._cb = None
+        self._pinned = []
         if planner is not None:
             self.AttachPlanner(planner)
 
@@ -127,6 +128,23 @@ class Simulator:
         if self._h:
             self._L.ecm_sim_destroy(self._h)
             self._h = C.c_void_p()
+        for p in getattr(self, "_pinned", []):
+            self._L.ecm_host_free(p)
+        self._pinned = []
+
+    def pinned(self, name: str) -> np.ndarray:
+        """A page-locked host buffer shaped for component `name`, for
+        read(name, out=...) every step: the copy then runs at full PCIe
+        speed.  The buffer belongs to the simulator and is freed by close();
+        do not use it afterwards."""
+        cid, dt, w = COMPONENTS[name]
+        shape = (self.max_agents, w) if w > 1 else (self.max_agents,)
+        nbytes = int(np.prod(shape)) * np.dtype(dt).itemsize
+        p = C.c_void_p()
+        N.check(self._L.ecm_host_alloc(nbytes, C.byref(p)))
+        self._pinned.append(p)
+        raw = (C.c_ubyte * nbytes).from_address(p.value)
+        return np.frombuffer(raw, dtype=dt).reshape(shape)
 
     def __del__(self):
         try:
@@ -223,10 +241,17 @@ class Simulator:
         N.check(self._L.ecm_sim_sync(self._h))
 
     # ---- getters (Simulator.h:114-126) ---------------------------------
-    def read(self, name: str) -> np.ndarray:
+    def read(self, name: str, out: np.ndarray = None) -> np.ndarray:
+        """Component `name` of every slot; into `out` when given (a buffer
+        kept from step to step, as a renderer keeps its own)."""
         cid, dt, w = COMPONENTS[name]
         shape = (self.max_agents, w) if w > 1 else (self.max_agents,)
-        a = np.empty(shape, dtype=dt)
+        if out is None:
+            a = np.empty(shape, dtype=dt)
+        else:
+            if out.shape != shape or out.dtype != np.dtype(dt) or not out.flags.c_contiguous:
+                raise ValueError(f"{name} needs a C-contiguous {np.dtype(dt)} array of shape {shape}")
+            a = out
         N.check(self._L.ecm_sim_read(self._h, cid, N.vptr(a), a.nbytes))
         return a
 
@@ -235,7 +260,7 @@ class Simulator:
         a = np.ascontiguousarray(a, dtype=dt)
         N.check(self._L.ecm_sim_write(self._h, cid, N.vptr(a), a.nbytes))
 
-    def GetPositionData(self): return self.read("position")
+    def GetPositionData(self, out=None): return self.read("position", out)
     def GetVelocityData(self): return self.read("velocity")
     def GetPreferredVelocityData(self): return self.read("pref_velocity")
     def GetAttractionPointData(self): return self.read("attraction")

