@@ -31,6 +31,7 @@
 namespace ecm {
 hipError_t launch_grid_build(const DevState& s, uint32_t* bsum, hipStream_t st);
 hipError_t launch_unsort(const DevState& s, hipStream_t st);
+hipError_t launch_unsort_pos(const DevState& s, hipStream_t st);
 hipError_t launch_mark_dead(const int* ids, int n, const int* pos_of, const float4* sorted_pr_old, uint8_t* dead_old,
                             const unsigned long long* acc, hipStream_t st);
 hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st, hipEvent_t mid);
@@ -167,6 +168,7 @@ struct ecm_sim {
     bool sorted_valid = false;                  // the last step's sorted outputs hold the state
     bool paths_resort = false;                  // the host rewrote path_off / path_cnt since the last step
     bool id_current = true;                     // the slot-indexed arrays hold it
+    bool pos_current = true;                    // the slot-indexed positions do (implied by id_current)
     DBuf<int> nbr[2], nbr_cnt[2];      // by step parity: the previous step's seed the search
     DBuf<int> fwd, prev_of, pos_of;    // previous <-> this step's sorted positions; id -> last one
     // host changes kept out of the slot-entry build: agents spawned and
@@ -1376,7 +1378,17 @@ static int sync_state(ecm_sim* s) {
     const DevState d = make_state(s);        // its "old" arrays are the last step's outputs
     HIPCHK(launch_unsort(d, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->id_current = true;
+    s->id_current = s->pos_current = true;
+    return 0;
+}
+
+// The slot-indexed arrays brought up to date on the stream, without waiting:
+// for a read-back that queues its copy behind the pass and then waits once.
+static int sync_state_queued(ecm_sim* s) {
+    if (s->id_current) return 0;
+    const DevState d = make_state(s);
+    HIPCHK(launch_unsort(d, s->stream));
+    s->id_current = s->pos_current = true;
     return 0;
 }
 
@@ -1692,7 +1704,7 @@ static int step_once(ecm_sim* s) {
     // the next step is ordered behind this one on the same stream.
     s->cur ^= 1;
     s->sorted_valid = true;                     // the state now lives in this step's sorted outputs
-    s->id_current = false;
+    s->id_current = s->pos_current = false;
     const int n_before = s->num_agents;
     s->last_grid_n = n_before;                // this step's grid held every agent active at its start
     std::sort(dead.begin(), dead.end());      // reference visits slots in ascending order
@@ -1874,7 +1886,16 @@ extern "C" int ecm_sim_read(ecm_sim* s, int c, void* dst, size_t bytes) {
     if (rc) return rc;
     if (bytes != el * (size_t)s->prm.max_agents) return fail(ECM_E_ARG, "component %d needs %zu bytes", c, el * (size_t)s->prm.max_agents);
     (void)hipSetDevice(s->dev);
-    if (int rc_ = sync_state(s)) return rc_;
+    // the slot-order pass and the copy queue behind the step; one wait.
+    // Positions alone need only the positions' pass.
+    if (c == ECM_C_POSITION && !s->id_current) {
+        if (!s->pos_current) {
+            HIPCHK(launch_unsort_pos(make_state(s), s->stream));
+            s->pos_current = true;
+        }
+    } else if (int rc_ = sync_state_queued(s)) {
+        return rc_;
+    }
     HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;

@@ -176,6 +176,9 @@ int ecm_sim_sync(ecm_sim* s);
 #define ECM_C_PREF_SPEED 5      /* float[max_agents]                         */
 #define ECM_C_ACTIVE 6          /* uint8[max_agents]                         */
 #define ECM_C_ORCA_VELOCITY 7   /* float[2 * max_agents] (keep_orca_velocity)*/
+/* Copies one component of every slot into dst (bytes must match).  Waits
+ * for the last step; reading positions alone brings only the positions
+ * into slot order. */
 int ecm_sim_read(ecm_sim* s, int component, void* dst, size_t bytes);
 /* Page-locked host memory for read-back buffers (no counterpart in the
  * reference, whose getters return its own host arrays).  ecm_sim_read into
