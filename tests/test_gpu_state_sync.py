@@ -69,3 +69,31 @@ def test_reads_see_the_last_step():
     ora = check_step(a, p, obs, ecm, "after reads")
     a.close()
     b.close()
+
+
+def test_position_reads_into_kept_and_page_locked_buffers():
+    """A renderer's read after every step: positions alone (the
+    positions-only slot-order pass), into a kept pageable buffer and into a
+    page-locked one from Simulator.pinned, equal a fresh read and, after the
+    whole-state pass of a later read of another component, still equal it;
+    a buffer of the wrong shape or type is refused."""
+    box, _, obs, ecm, agents = small_scene(63, half=30.0, n_rects=4, n_agents=900)
+    p = Params(max_agents=1024)
+    sim = Simulator(ecm, obs, box, p)
+    sim.SpawnAgents(agents)
+    kept = sim.GetPositionData()
+    pinned = sim.pinned("position")
+    for _ in range(4):
+        sim.Update()
+        sim.GetPositionData(out=pinned)              # positions only
+        sim.GetPositionData(out=kept)
+        vel = sim.read("velocity")                   # the whole state
+        fresh = sim.GetPositionData()
+        assert np.array_equal(pinned, fresh)
+        assert np.array_equal(kept, fresh)
+        assert vel.shape == (1024, 2)
+    with pytest.raises(ValueError):
+        sim.GetPositionData(out=np.zeros((1024, 3), np.float32))
+    with pytest.raises(ValueError):
+        sim.GetPositionData(out=np.zeros((1024, 2), np.float64))
+    sim.close()
