@@ -92,6 +92,11 @@ static GridDesc make_grid(float x0, float y0, float x1, float y1, float cell, in
 }
 
 // One environment on the host: its scene, device copies and agent grid.
+// Longest obstacle bucket list a scene may have: obstacle edges within the
+// obstacle range of one point.  It sizes the 3D-LP queue's constraint stride
+// (kMaxBucketEdges + k constraints of 16 B per agent slot at most).
+constexpr int kMaxBucketEdges = 512;
+
 struct EnvHost {
     float x0 = 0, y0 = 0, x1 = 0, y1 = 0;       // world box
     // obstacles
@@ -209,7 +214,8 @@ struct ecm_sim {
     DBuf<EnvDesc> d_envs;
     bool envs_dirty = true;
     uint32_t n_keys = 0;
-    int maxo = 16;                 // longest obstacle bucket list of any environment (16 / 32 / 64)
+    int maxo = 16;                 // longest obstacle bucket list of any environment (16 / 32 / 64, then
+                                   // multiples of 16 up to kMaxBucketEdges)
     int n_cu = 256;                // compute units of the device (persistent queue grids)
     float max_radius = 0.0f, max_pref_speed = 0.0f;   // over every agent spawned
     DBuf<uint16_t> agent_env;      // environment of each slot (with more than one environment)
@@ -539,6 +545,7 @@ static int build_obstacle_buckets(ecm_sim* s, EnvHost* E, float range) {
     const float margin = 0.05f;
     const double R = (double)range + margin;
     float cell = 2.0f;
+    size_t prev_maxlen = SIZE_MAX;
     for (int attempt = 0; attempt < 6; ++attempt, cell *= 0.5f) {
         GridDesc g = make_grid(E->x0, E->y0, E->x1, E->y1, cell, 4096);
         const int side = g.side;
@@ -572,8 +579,16 @@ static int build_obstacle_buckets(ecm_sim* s, EnvHost* E, float range) {
         }
         size_t maxlen = 0, total = 0;
         for (auto& l : lists) { maxlen = std::max(maxlen, l.size()); total += l.size(); }
-        if (maxlen > 64 && attempt < 5) continue;
-        if (maxlen > 64) return fail(ECM_E_STATE, "more than 64 obstacle edges within range of one point");
+        // Finer cells shorten the lists while the cells are large against
+        // the obstacle range.  Once halving the cell takes less than a tenth
+        // off the longest list, the edges within range of single points
+        // dominate it and finer cells only cost memory: the overflow passes
+        // then take a tile sized for the longest list (up to kMaxBucketEdges).
+        const bool shrinking = prev_maxlen == SIZE_MAX || maxlen * 10 < prev_maxlen * 9;
+        prev_maxlen = maxlen;
+        if (maxlen > 64 && attempt < 5 && shrinking) continue;
+        if (maxlen > (size_t)kMaxBucketEdges)
+            return fail(ECM_E_STATE, "more than %d obstacle edges within range of one point", kMaxBucketEdges);
         std::vector<int> start((size_t)side * side + 1), edges;
         edges.reserve(total);
         for (size_t c = 0; c < lists.size(); ++c) {
@@ -590,7 +605,7 @@ static int build_obstacle_buckets(ecm_sim* s, EnvHost* E, float range) {
         HIPCHK(hipStreamSynchronize(s->stream));
         E->og = g;
         s->envs_dirty = true;
-        E->maxo = maxlen <= 16 ? 16 : (maxlen <= 32 ? 32 : 64);
+        E->maxo = maxlen <= 16 ? 16 : (maxlen <= 32 ? 32 : (maxlen <= 64 ? 64 : (int)((maxlen + 15) & ~(size_t)15)));
         E->obst_range_built = range;
         return 0;
     }

@@ -1723,9 +1723,9 @@ __device__ __forceinline__ void orca_reg_agent(const DevState& s, int t) {
 // neighbour per lane.  Each constraint's arithmetic, the LP and what follows
 // it are those of orca_agent in the plain mode, so the result is the same;
 // a lane working through an agent alone waits out every constraint's
-// loads and arithmetic in turn.  lds holds MAXO + K constraints at slot c.
-template <int K, int MAXO>
-__device__ __forceinline__ void orca_agent_wave(const DevState& s, int t, float4* lds) {
+// loads and arithmetic in turn.  lds holds maxo + K constraints at slot c.
+template <int K>
+__device__ __forceinline__ void orca_agent_wave(const DevState& s, int t, float4* lds, int maxo) {
     const int lane = threadIdx.x;
     const Tile T{0, 0, 0};                           // slot c at lds[c]
     const float4 pr = s.sorted_pr[t];
@@ -1758,10 +1758,10 @@ __device__ __forceinline__ void orca_agent_wave(const DevState& s, int t, float4
                 made = obstacle_constraint(s, E, e, position, velocity, rad, cp, cn);
             const unsigned long long m = __ballot(made);
             const int slot = nObst + __popcll(m & ((1ull << lane) - 1ull));
-            if (made && slot < MAXO) con_st(lds, T, slot, cp, cn);
+            if (made && slot < maxo) con_st(lds, T, slot, cp, cn);
             nObst += __popcll(m);
         }
-        nObst = min(nObst, MAXO);                    // MAXO covers the longest bucket
+        nObst = min(nObst, maxo);                    // maxo covers the longest bucket
     }
     const int cnt = s.nbr_cnt[t];
     if (lane < cnt) {
@@ -1801,7 +1801,7 @@ __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     const uint32_t nq = s.stats[1];
     const int tid = threadIdx.x;
     if (nq <= gridDim.x) {
-        if (blockIdx.x < nq) orca_agent_wave<K, MAXO>(s, s.ovf_agent[blockIdx.x], lds);
+        if (blockIdx.x < nq) orca_agent_wave<K>(s, s.ovf_agent[blockIdx.x], lds, MAXO);
         return;
     }
     // entries go lane-major across the blocks (entry i: lane i / gridDim,
@@ -1809,6 +1809,16 @@ __global__ __launch_bounds__(64) void k_orca_ovf(DevState s) {
     // filling the lanes of a few divergent wavefronts
     for (uint32_t i = tid * gridDim.x + blockIdx.x; i < nq; i += gridDim.x * 64)
         orca_agent<K, MAXO, false, kPlain>(s, s.ovf_agent[i], lds, Tile{tid, INT_MAX, 0}, nullptr);
+}
+
+// The overflow pass for scenes whose obstacle buckets hold more than 64
+// edges: one agent per wavefront for any queue length, the tile of maxo + K
+// constraints in dynamic LDS (one agent's, not 64 lanes').
+template <int K>
+__global__ __launch_bounds__(64) void k_orca_ovf_wide(DevState s, int maxo) {
+    extern __shared__ float4 wide_lds[];
+    const uint32_t nq = s.stats[1];
+    for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) orca_agent_wave<K>(s, s.ovf_agent[i], wide_lds, maxo);
 }
 
 // RandomizedLP3D (ORCA.cpp:592-669) with the constraints in an LDS tile,
@@ -1834,6 +1844,35 @@ __global__ __launch_bounds__(64) void k_lp3d(DevState s) {
         V2 o = mk(ov.x, ov.y);
         lp3d(lds, T, info.x, info.y, MAXO + K, maxSpeed, info.z, o);
         integrate(s, t, mk(pr.x, pr.y), mk(sv.x, sv.y), o);
+    }
+}
+
+// k_lp3d for scenes whose obstacle buckets hold more than 64 edges: one
+// queue entry per wavefront, its constraints copied by the lanes into a tile
+// of maxo + 2K slots in dynamic LDS (the projected constraints from maxo + K
+// on), and lane 0 runs the same lp3d.
+template <int K>
+__global__ __launch_bounds__(64) void k_lp3d_wide(DevState s, int maxo) {
+    extern __shared__ float4 wide_lds[];
+    const uint32_t nq = s.stats[4];
+    const int lane = threadIdx.x;
+    const Tile T{0, 0, 0};                           // slot c at wide_lds[c]
+    for (uint32_t i = blockIdx.x; i < nq; i += gridDim.x) {
+        const uint32_t q = (uint32_t)s.q3d_big[i];
+        const int4 info = s.q3d_info[q];
+        for (int c = lane; c < info.x; c += 64) wide_lds[c] = *q3d_slot(s, q, c);
+        __syncthreads();
+        if (lane == 0) {
+            const int t = s.q3d_agent[q];
+            const float4 pr = s.sorted_pr[t];
+            const float4 sv = s.sorted_v[t];
+            const float maxSpeed = s.max_speed_from_pref ? sv.z : s.max_speed;
+            const float2 ov = s.q3d_vel[q];
+            V2 o = mk(ov.x, ov.y);
+            lp3d(wide_lds, T, info.x, info.y, maxo + K, maxSpeed, info.z, o);
+            integrate(s, t, mk(pr.x, pr.y), mk(sv.x, sv.y), o);
+        }
+        __syncthreads();                             // the tile is reused by the next entry
     }
 }
 
@@ -2039,10 +2078,15 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int
         hipLaunchKernelGGL((k_orca_ovf<K, 32>), gq, b, 0, st, s);
         hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 32>), gq, b, 0, st, s);
-    } else {
+    } else if (maxo <= 64) {
         hipLaunchKernelGGL((k_orca_ovf<K, 64>), gq, b, 0, st, s);
         hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
         hipLaunchKernelGGL((k_lp3d<K, 64>), gq, b, 0, st, s);
+    } else {
+        // one agent per wavefront, the tile sized for maxo in dynamic LDS
+        hipLaunchKernelGGL((k_orca_ovf_wide<K>), gq, b, (size_t)(maxo + K) * sizeof(float4), st, s, maxo);
+        hipLaunchKernelGGL((k_lp3d_reg<K, kLp3dRegObst>), gr, br, 0, st, s);
+        hipLaunchKernelGGL((k_lp3d_wide<K>), gq, b, (size_t)(maxo + 2 * K) * sizeof(float4), st, s, maxo);
     }
 }
 

@@ -127,3 +127,44 @@ def test_parity_pillar_forest_long_overflow_queue():
     sim.close()
     assert longest > 1024, longest          # the kernel's grid: 1,024 wavefronts
     assert all(v == 0 for v in worst.values()), worst
+
+
+def test_parity_obstacle_buckets_longer_than_64_edges():
+    """0.5 m squares on a 1.6 m lattice: up to 194 obstacle edges lie within
+    the obstacle range of one point, more than the 64 that the overflow
+    kernels' fixed tiles hold, so the scene takes the overflow passes with
+    a tile sized for the longest bucket list (one agent per wavefront);
+    agents have up to 56 obstacle constraints.  Every agent must match the
+    oracle bit for bit, the 3D LP included."""
+    from oracle import oracle_py as O
+    from ecmgenerator_amd import scenarios as S
+    half = 16.0
+    box = (-half, -half, half, half)
+    rects = [(x, y, x + 0.5, y + 0.5) for x in np.arange(-half + 2, half - 2, 1.6)
+             for y in np.arange(-half + 2, half - 2, 1.6)]
+    obs = S.rects_to_obstacles(rects)
+    ecm = S.empty_square_ecm(*box)
+    rng = np.random.default_rng(7)
+    agents = S.straight_agents(rng, box, 300, rects, margin=0.4)
+    p = Params(max_agents=320, keep_orca_velocity=True)
+    sim = Simulator(ecm, obs, box, p)
+    assert (sim.SpawnAgents(agents) >= 0).sum() > 250
+    worst = {}
+    used_3d = 0
+    most = 0
+    for it in range(4):
+        before = oracle_state(sim)
+        ora = O.step(p, before, obs, ecm)
+        sim.Update()
+        res = compare_step(sim, before, ora)
+        act = before["active"].astype(bool)
+        used_3d += int(ora["used_3d"][act].sum())
+        most = max(most, int(ora["diag"][act, 1].max()))
+        print(f"wide buckets step={it} {res} most obstacle constraints {most}", flush=True)
+        for k, v in res.items():
+            worst[k] = max(worst.get(k, 0), v)
+    st = sim.stats()
+    sim.close()
+    assert most > 16 and st["obst_overflow"] > 0, (most, st)
+    assert st["lp3d"] == used_3d and used_3d > 0, (st, used_3d)
+    assert all(v == 0 for v in worst.values()), worst
