@@ -257,7 +257,7 @@ struct ecm_sim {
     ecm_mlp* log_mlp = nullptr;     // set for the duration of ecm_sim_log_step
 
     // launch shape (results do not depend on it; ecm_sim_autotune picks it)
-    ecm_sim_tuning tune{1.25f, 256, 1, 0.0f};
+    ecm_sim_tuning tune{1.25f, 256, 2, 0.0f};   // the shape the autotuner picks at 100k-2M agents
     // the device part of a step captured as two hipGraphs (up to the replan
     // read-back, and the rest), one pair per position buffer parity,
     // recaptured when anything they were captured with changes

@@ -337,7 +337,7 @@ class Simulator:
         return {"cell_scale": t.cell_scale, "knn_block": t.knn_block, "orca_obst_slots": t.orca_obst_slots,
                 "best_ms": t.best_ms}
 
-    def set_tuning(self, cell_scale=1.25, knn_block=256, orca_obst_slots=1) -> None:
+    def set_tuning(self, cell_scale=1.25, knn_block=256, orca_obst_slots=2) -> None:
         t = N.SimTuning(float(cell_scale), int(knn_block), int(orca_obst_slots), 0.0)
         N.check(self._L.ecm_sim_set_tuning(self._h, C.byref(t)))
 
