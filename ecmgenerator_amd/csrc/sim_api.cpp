@@ -160,6 +160,7 @@ struct ecm_sim {
     uint64_t repack_expect = 0;
     bool repack_pending_check = false;
     int steps_since_repack = 0;
+    bool repack_soon = false;                // the pool was laid out in slot order (load, compaction)
     int cur = 0;
 
     // agent grid
@@ -872,6 +873,7 @@ static int compact_paths(ecm_sim* s) {
     s->path_used = out.size();
     ++s->path_compactions;
     s->paths_resort = true;        // the last step's sorted copy holds the old offsets
+    s->repack_soon = true;         // the pool is in slot order now
     return 0;
 }
 
@@ -1062,7 +1064,10 @@ static constexpr int kPathRepackSteps = 64;
 
 static int repack_paths(ecm_sim* s) {
     if (s->repack_pending_check) {
-        // long finished: the stream has run 64 steps since
+        // long finished: the stream has run 64 steps since, unless this
+        // repack comes early (after a load or a compaction): then wait
+        if (s->steps_since_repack < kPathRepackSteps)
+            if (int rc = settle(s)) return rc;
         if (s->h_repack[1] != 0u || s->h_repack[0] != (uint32_t)s->repack_expect)
             return fail(ECM_E_STATE, "path repack packed %u points (expected %llu, skipped %u)", s->h_repack[0],
                         (unsigned long long)s->repack_expect, s->h_repack[1]);
@@ -1114,9 +1119,12 @@ static int upload_pending(ecm_sim* s) {
     }
     // every live agent is in the last step's sorted outputs unless some were
     // spawned since
-    if (++s->steps_since_repack >= kPathRepackSteps && s->pending_new.empty()) {
+    // also at the first step after a snapshot load or a compaction, which
+    // leave the pool in slot order
+    if ((++s->steps_since_repack >= kPathRepackSteps || s->repack_soon) && s->pending_new.empty()) {
         if (int rc = repack_paths(s)) return rc;
         s->steps_since_repack = 0;
+        s->repack_soon = false;
     }
     if (!s->pending_new.empty()) {
         s->new_stage.swap(s->pending_new);
@@ -2323,5 +2331,6 @@ static int snapshot_load_impl(ecm_sim* s, const char* path) {
         s->h_path_cnt[i] = cnt[i];
         s->h_goal[i] = cnt[i] > 0 ? pool[off[i] + cnt[i] - 1] : make_float2(0.0f, 0.0f);
     }
+    s->repack_soon = true;                      // the file's pool is in slot order
     return 0;
 }
