@@ -1785,7 +1785,8 @@ extern "C" int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap)
 
 static int check_tuning(const ecm_sim_tuning& t) {
     if (!(t.cell_scale >= 0.25f && t.cell_scale <= 4.0f)) return fail(ECM_E_ARG, "cell_scale must be in [0.25, 4]");
-    if (t.knn_block != 64 && t.knn_block != 128 && t.knn_block != 256) return fail(ECM_E_ARG, "knn_block must be 64, 128 or 256");
+    if (t.knn_block != 16 && t.knn_block != 64 && t.knn_block != 128 && t.knn_block != 256)
+        return fail(ECM_E_ARG, "knn_block must be 16, 64, 128 or 256");
     if (t.orca_obst_slots != 1 && t.orca_obst_slots != 2 && t.orca_obst_slots != 4)
         return fail(ECM_E_ARG, "orca_obst_slots must be 1, 2 or 4");
     return 0;
@@ -1848,11 +1849,11 @@ extern "C" int ecm_sim_autotune(ecm_sim* s, int steps, ecm_sim_tuning* out) {
     float best_ms = 0.0f;
     int rc = 0;
     const float scales[] = {0.6f, 0.8f, 1.0f, 1.25f, 1.6f, 2.0f};
-    const int blocks[] = {64, 128, 256};
+    const int blocks[] = {64, 128, 256, 16};
     const int slots[] = {1, 2, 4};
     for (int dim = 0; dim < 3; ++dim) {
         std::vector<ecm_sim_tuning> opts{best};
-        const int nopt = dim == 0 ? 6 : 3;
+        const int nopt = dim == 0 ? 6 : (dim == 1 ? 4 : 3);
         for (int o = 0; o < nopt; ++o) {
             ecm_sim_tuning t = best;
             if (dim == 0) t.cell_scale = scales[o];

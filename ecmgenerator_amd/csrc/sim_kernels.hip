@@ -1612,6 +1612,24 @@ __global__ __launch_bounds__(64, orca_reg_waves(K, OB)) void k_orca_reg(DevState
 // waited for.  One kernel instead of two: the neighbour list is read back
 // from the thread's own writes, and the tail of one pass overlaps the
 // other's.  Capped at 80 VGPRs for 6 waves per SIMD.
+// k_knn_orca for small crowds: 16 agents per 64-lane wavefront.  A crowd
+// of a few thousand agents fills a few dozen wavefronts, one per SIMD, and
+// the pass lasts as long as one wavefront's walk, constraints and LP; with
+// 16 lanes a wavefront runs the union of 16 lanes' loops instead of 64, on
+// four times as many SIMDs (c0: 66 -> ~48 us).  The launch shape's
+// knn_block 16 selects it; the LDS is laid out for 64 lanes as in
+// k_knn_orca.
+template <int K>
+__global__ __launch_bounds__(64, 6) void k_knn_orca_sparse(DevState s) {
+    extern __shared__ int knn_lds[];
+    const uint32_t n_active = s.cell_start[s.n_keys];
+    if (threadIdx.x >= 16) return;
+    const int t = xcd_block() * 16 + threadIdx.x;
+    if (t >= (int)n_active) return;
+    knn_agent<K>(s, knn_lds, t);
+    orca_reg_agent<K, 4>(s, t);
+}
+
 template <int K>
 __global__ __launch_bounds__(256, 6) void k_knn_orca(DevState s) {
     extern __shared__ int knn_lds[];
@@ -2095,7 +2113,15 @@ static void launch_orca_k(const DevState& s, int maxo, int mode, int oslots, int
 hipError_t launch_attr_knn(const DevState& s, int block, int mode, int oslots, hipStream_t st, hipEvent_t mid) {
     hipLaunchKernelGGL(k_attr, dim3(round8(cdiv(s.n_slots, 256))), dim3(256), 0, st, s);
     if (mid) (void)hipEventRecord(mid, st);
-    if (block != 64 && block != 128) block = 256;
+    if (block == 16 && orca_fused(s.k, mode, oslots)) {
+        // 16 agents per wavefront of 64 threads
+        const dim3 gs(round8(cdiv(s.n_slots, 16))), bs(64);
+        const size_t lds64 = (size_t)(kKnnBuf + 2 * kKnnRanges) * 64 * sizeof(int);
+        if (s.k == 5) hipLaunchKernelGGL((k_knn_orca_sparse<5>), gs, bs, lds64, st, s);
+        else hipLaunchKernelGGL((k_knn_orca_sparse<10>), gs, bs, lds64, st, s);
+        return hipGetLastError();
+    }
+    if (block != 64 && block != 128) block = block == 16 ? 64 : 256;
     dim3 g(round8(cdiv(s.n_slots, block))), b(block);
     const size_t lds = (size_t)(kKnnBuf + 2 * kKnnRanges) * block * sizeof(int);
     if (orca_fused(s.k, mode, oslots)) {

@@ -114,7 +114,9 @@ int ecm_sim_graph_status(ecm_sim* s, int* active, char* why, int cap);
 /* Launch shape; no choice changes any result.  cell_scale multiplies the
  * agent grid cell edge (about one agent per cell at 1; default 1.25),
  * knn_block is the workgroup size of the k-nearest kernel (64,
- * 128, 256; default 256), and orca_obst_slots sizes the main ORCA pass (1,
+ * 128, 256; default 256; 16: 16 agents per 64-lane wavefront of the fused
+ * kernel below, for crowds of a few thousand agents, and 64 otherwise),
+ * and orca_obst_slots sizes the main ORCA pass (1,
  * 2, 4; default 2): the plain step holds 4 obstacle constraints per agent
  * in registers (8 with orca_obst_slots 4 and k <= 10) and sends agents with
  * more to the overflow pass; with orca_obst_slots 2 and k 5 or 10 that pass

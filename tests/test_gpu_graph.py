@@ -76,7 +76,7 @@ def test_autotune_changes_speed_not_results():
     b = make(sc, agents, True)
     t = b.autotune(steps=2)
     print(f"autotune picked {t}", flush=True)
-    assert t["knn_block"] in (64, 128, 256) and t["orca_obst_slots"] in (1, 2, 4) and t["best_ms"] > 0
+    assert t["knn_block"] in (16, 64, 128, 256) and t["orca_obst_slots"] in (1, 2, 4) and t["best_ms"] > 0
     n = b.stats()["steps"]
     a.Update(steps=n)
     same(a, b)
@@ -91,16 +91,18 @@ def test_autotune_changes_speed_not_results():
 @pytest.mark.parametrize("k", [5, 10])
 def test_orca_pass_shapes_agree(k):
     """The main ORCA pass as its own kernel with 4 obstacle slots (shape 1),
-    inside the attraction / k-nearest kernel (shape 2, k 5 and 10) and with
-    8 obstacle slots (shape 4) gives the same steps bit for bit, with step
-    graphs and eager launches, replans included."""
+    inside the k-nearest kernel (shape 2, k 5 and 10; also with 16 agents per
+    wavefront, knn_block 16) and with 8 obstacle slots (shape 4) gives the
+    same steps bit for bit, with step graphs and eager launches, replans
+    included."""
     sc, agents, _ = generate(0, seed=7)
     sims = []
-    for oslots, graph in ((1, False), (2, True), (2, False), (4, True)):
+    for oslots, graph, block in ((1, False, 256), (2, True, 256), (2, False, 256), (4, True, 256), (2, True, 16),
+                                 (1, True, 16)):
         p = Params(max_agents=agents.n + 64, keep_orca_velocity=True, k=k)
         sim = Simulator(sc.ecm(), sc.obstacles(), sc.box, p, planner=sc)
         sim.SpawnAgents(agents)
-        sim.set_tuning(orca_obst_slots=oslots)
+        sim.set_tuning(knn_block=block, orca_obst_slots=oslots)
         sim.set_graph(graph)
         sims.append(sim)
     for step in range(40):
